@@ -1,0 +1,287 @@
+"""Pins the CPU oracle before anything is compared with it.
+
+Each oracle function is checked against an independent source:
+  distances  scikit-learn's euclidean DistanceMetric (fp64) and NumPy's
+             element-wise float32 arithmetic (fp32)
+  core       scikit-learn KD-tree k-th neighbour distances (fp64)
+  prim       scikit-learn mst_from_data_matrix (fp64, bitwise) and the
+             reference's brute-force sequential Prim on hand-built matrices
+             (reference tests/test_kernels.py:383-399, 782-803); MST total
+             weight vs SciPy's minimum_spanning_tree
+  argsort    np.argsort(kind="quicksort") with SIMD sorting disabled
+             (fixtures), 5 reference patterns x sizes 2..4096
+  linkage    scikit-learn make_single_linkage (fp64, bitwise)
+  condense   scikit-learn _condense_tree rows (fp64, bitwise)
+  select     scikit-learn labels (exact) on every reference corpus and
+             workload window, and the reference's known answers
+Planted mistakes (stable sort, non-strict Prim update, swapped linkage
+children) are shown to fail these checks.
+"""
+
+import numpy as np
+import pytest
+
+from golden_io import records, same_partition, workload_windows
+from oracle import oracle
+
+STAGES = records("stages_f64")
+CORPUS = records("corpus")
+ARGSORT = records("argsort")
+WORKLOADS = workload_windows()
+
+
+def test_oracle_exports_every_function():
+    lib = oracle.lib()
+    for base in ("distances", "core", "prim", "argsort", "linkage", "tree", "hdbscan"):
+        for sfx in ("f32", "f64"):
+            assert hasattr(lib, f"orc_{base}_{sfx}")
+
+
+@pytest.mark.parametrize("index", range(len(ARGSORT)), ids=[r["name"] for r in ARGSORT])
+def test_argsort_matches_numpy_scalar_quicksort(index):
+    rec = ARGSORT[index]
+    values = rec["w64"].astype(np.float32 if rec["is32"] else np.float64)
+    np.testing.assert_array_equal(oracle.argsort(values), rec["order"])
+
+
+def test_planted_stable_sort_fails_argsort_fixtures():
+    mismatches = 0
+    for rec in ARGSORT:
+        values = rec["w64"].astype(np.float32 if rec["is32"] else np.float64)
+        mismatches += not np.array_equal(np.argsort(values, kind="stable"), rec["order"])
+    assert mismatches > 10
+
+
+@pytest.mark.parametrize("index", range(len(STAGES)))
+def test_fp64_stages_match_sklearn_bitwise(index):
+    from sklearn.metrics import DistanceMetric
+
+    r = STAGES[index]
+    x, n, mcs, ms, eps = r["x"], r["n"], r["mcs"], r["ms"], r["eps"]
+    dist = oracle.distances(x, "float64")
+    expected = DistanceMetric.get_metric("euclidean").pairwise(x)
+    np.testing.assert_array_equal(dist.view(np.uint64), expected.view(np.uint64))
+    core = oracle.core(dist, ms, "float64")
+    np.testing.assert_array_equal(core.view(np.uint64), r["core"].view(np.uint64))
+    u, v, w = oracle.prim(dist, core, unit_core=False, precision="float64")
+    np.testing.assert_array_equal(u, r["mst_u"])
+    np.testing.assert_array_equal(v, r["mst_v"])
+    np.testing.assert_array_equal(w.view(np.uint64), r["mst_w"].view(np.uint64))
+    if ms <= 2:
+        uu, vv, ww = oracle.prim(dist, None, unit_core=True, precision="float64")
+        np.testing.assert_array_equal(uu, u)
+        np.testing.assert_array_equal(vv, v)
+        np.testing.assert_array_equal(ww, w)
+    t = oracle.tree(u, v, w, mcs, eps, precision="float64")
+    np.testing.assert_array_equal(t["order"], r["mst_order"])
+    np.testing.assert_array_equal(t["left"], r["slt_left"])
+    np.testing.assert_array_equal(t["right"], r["slt_right"])
+    np.testing.assert_array_equal(t["sizes"], r["slt_size"])
+    np.testing.assert_array_equal(t["distance"].view(np.uint64), r["slt_value"].view(np.uint64))
+    # Condensed tree rows -> cluster parents, births and point parents.
+    parent, child, value, size = r["ct_parent"], r["ct_child"], r["ct_value"], r["ct_size"]
+    clusters = child >= n
+    nl = t["next_label"]
+    assert nl == (child[clusters].max() + 1 if clusters.any() else n + 1)
+    np.testing.assert_array_equal(t["parent"][child[clusters] - n], parent[clusters])
+    np.testing.assert_array_equal(t["birth"][child[clusters] - n].view(np.uint64),
+                                  value[clusters].view(np.uint64))
+    np.testing.assert_array_equal(t["point_parent"][child[~clusters]], parent[~clusters])
+    # Stability as scikit-learn accumulates it over its own condensed rows,
+    # in row order; EOM then overwrites losing clusters with subtree sums, so
+    # compare only clusters that stayed selected or are leaves of the tree.
+    births = np.zeros(max(nl, n + 1))
+    births[child[clusters]] = value[clusters]
+    stab = np.zeros(nl - n)
+    for p, val, sz in zip(parent, value, size):
+        stab[p - n] += (val - births[p]) * sz
+    assert np.array_equal(t["labels"], r["labels"])
+    has_children = np.zeros(nl - n, bool)
+    has_children[parent[clusters] - n] = True
+    leaf = ~has_children
+    np.testing.assert_array_equal(t["stability"][leaf].view(np.uint64), stab[leaf].view(np.uint64))
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_corpus_labels_match_sklearn(precision):
+    failures = []
+    for r in CORPUS:
+        labels, status = oracle.hdbscan(r["x"], r["mcs"], r["ms"], r["eps"], precision=precision)
+        assert status == 0
+        if not np.array_equal(labels, r["labels"]):
+            failures.append(r["name"])
+    # Every corpus case of the reference's label tests agrees exactly; the one
+    # documented float32 case is the epsilon just above 1/8, which rounds to
+    # 1/8 in float32 (reference tests/test_parity.py:244-263).
+    allowed = {"float32": {"boundary:0.12500000000000003"}, "float64": set()}[precision]
+    assert set(failures) <= allowed, failures
+
+
+@pytest.mark.parametrize("index", range(len(WORKLOADS)), ids=[w[0] for w in WORKLOADS])
+def test_workload_labels_match_sklearn(index):
+    name, x, precision, mcs, ms, eps, expected = WORKLOADS[index]
+    if x.shape[0] * x.shape[1] > 2048 * 32:
+        pytest.skip("large window: covered by the GPU parity test")
+    labels, status = oracle.hdbscan(x, mcs, ms, eps, precision=precision)
+    assert status == 0
+    assert same_partition(labels, expected)
+
+
+def sequential_prim(dist, core):
+    """Brute-force Prim of the reference's tests (tests/test_kernels.py:383-399)."""
+    size = len(core)
+    visited = np.zeros(size, bool)
+    best = np.full(size, np.inf, dist.dtype)
+    sources = np.ones(size, np.int64)
+    edges = []
+    current = 0
+    for _ in range(size - 1):
+        visited[current] = True
+        weights = np.maximum(np.maximum(core[current], core), dist[current])
+        improved = (weights < best) & ~visited
+        best[improved] = weights[improved]
+        sources[improved] = current
+        nxt = int(np.argmin(np.where(visited, np.inf, best)))
+        edges.append((sources[nxt], nxt, best[nxt]))
+        current = nxt
+    return np.asarray(edges)
+
+
+def prim_matrices(size, dtype):
+    rng = np.random.default_rng(size)
+    equal = np.ones((size, size), dtype)
+    retained = np.full((size, size), 9, dtype)
+    if size >= 4:
+        for a, b, value in [(0, 1, 1), (0, 2, 2), (1, 3, 3), (2, 3, 3)]:
+            retained[a, b] = retained[b, a] = value
+    quantized = rng.integers(1, 7, size=(size, size)).astype(dtype)
+    quantized = np.minimum(quantized, quantized.T)
+    mats = np.stack([equal, retained, quantized])
+    for m in mats:
+        np.fill_diagonal(m, 0)
+    core = np.zeros((3, size), dtype)
+    core[2] = rng.integers(0, 4, size=size)
+    return mats, core
+
+
+@pytest.mark.parametrize("size", [2, 5, 64, 255, 257])
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_prim_matches_brute_force_and_mst_weight(size, precision):
+    from scipy.sparse.csgraph import minimum_spanning_tree
+
+    dtype = np.float32 if precision == "float32" else np.float64
+    mats, core = prim_matrices(size, dtype)
+    for m, c in zip(mats, core):
+        u, v, w = oracle.prim(m, c, precision=precision)
+        exp = sequential_prim(m, c)
+        np.testing.assert_array_equal(u, exp[:, 0])
+        np.testing.assert_array_equal(v, exp[:, 1])
+        np.testing.assert_array_equal(w, exp[:, 2].astype(dtype))
+        mr = np.maximum(np.maximum(c[:, None], c[None, :]), m).astype(np.float64)
+        np.fill_diagonal(mr, 0)
+        # csgraph treats zeros as missing edges; shift weights by one.
+        total = minimum_spanning_tree(mr + 1 - np.eye(size)).sum() - (size - 1)
+        assert abs(total - w.astype(np.float64).sum()) <= 1e-9 * max(1.0, abs(total))
+
+
+def test_planted_nonstrict_prim_update_fails_brute_force():
+    mats, core = prim_matrices(64, np.float64)
+    m, c = mats[1], core[1]
+    exp = sequential_prim(m, c)
+    # Non-strict update: equal weights move the source to the newer vertex.
+    size = len(c)
+    visited = np.zeros(size, bool)
+    best = np.full(size, np.inf)
+    src = np.ones(size, np.int64)
+    cur, edges = 0, []
+    for _ in range(size - 1):
+        visited[cur] = True
+        wts = np.maximum(np.maximum(c[cur], c), m[cur])
+        imp = (wts <= best) & ~visited
+        best[imp], src[imp] = wts[imp], cur
+        nxt = int(np.argmin(np.where(visited, np.inf, best)))
+        edges.append((src[nxt], nxt))
+        cur = nxt
+    assert not np.array_equal(np.asarray(edges), exp[:, :2].astype(np.int64))
+
+
+def test_planted_swapped_children_fails_linkage_fixture():
+    r = STAGES[0]
+    dist = oracle.distances(r["x"], "float64")
+    core = oracle.core(dist, r["ms"], "float64")
+    u, v, w = oracle.prim(dist, core, precision="float64")
+    t = oracle.tree(v, u, w, r["mcs"], r["eps"], precision="float64")
+    assert not np.array_equal(t["left"], r["slt_left"])
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_distances_closed_form(precision):
+    rng = np.random.default_rng(7)
+    dtype = np.float32 if precision == "float32" else np.float64
+    for n, d in ((5, 1), (33, 8), (129, 31)):
+        x = rng.normal(size=(n, d)).astype(dtype)
+        x[0] = 0
+        x[-1] = -0.0
+        x[1] = x[0]
+        x[2] *= dtype(1e9)
+        x[3] = x[4] + dtype(1e-30)  # differences in the float32 subnormal range
+        dist = oracle.distances(x, precision)
+        total = np.zeros((n, n), dtype)
+        for k in range(d):
+            delta = x[:, None, k] - x[None, :, k]
+            total = total + delta * delta
+        expected = np.sqrt(total)
+        bits = np.uint32 if precision == "float32" else np.uint64
+        np.testing.assert_array_equal(dist.view(bits), expected.view(bits))
+
+
+def test_core_is_order_statistic():
+    rng = np.random.default_rng(3)
+    x = rng.integers(-4, 5, (65, 3)).astype(np.float32)
+    dist = oracle.distances(x, "float32")
+    for ms in (1, 2, 3, 7, 65):
+        np.testing.assert_array_equal(oracle.core(dist, ms), np.partition(dist, ms - 1, axis=-1)[:, ms - 1])
+
+
+# Known answers from the reference's own tests.
+GAP = 2.0 ** -30
+COLLAPSING = np.repeat(1 + np.array([0, GAP, 16 * GAP, 17 * GAP]), 2)[:, None]
+BOUNDARY_POINTS = np.repeat(1 + 2.0 ** -40 + np.array([0, 0.125, 4, 4.125]), 2)[:, None]
+BOUNDARY = np.array([[0], [0], [0.125], [0.125], [4], [4], [4.125], [4.125]], np.float32)
+
+
+def test_known_answers_float64():
+    labels, _ = oracle.hdbscan(COLLAPSING, 2, 1, 0.0, "float64")
+    assert same_partition(labels, np.repeat(np.arange(4), 2))
+    labels, _ = oracle.hdbscan(COLLAPSING.astype(np.float32), 2, 1, 0.0, "float32")
+    assert same_partition(labels, np.full(8, -1))
+    for eps, clusters in ((float(np.nextafter(0.125, 0)), 4), (0.125, 4), (float(np.nextafter(0.125, 1)), 2)):
+        labels, _ = oracle.hdbscan(BOUNDARY_POINTS, 2, 1, eps, "float64")
+        assert same_partition(labels, np.repeat(np.arange(clusters), 8 // clusters))
+
+
+def test_known_answers_float32():
+    for eps, clusters in ((0.0, 4), (0.1, 4), (float(np.nextafter(np.float32(0.125), np.float32(0))), 4),
+                          (0.125, 4), (float(np.nextafter(np.float32(0.125), np.float32(1))), 2)):
+        labels, _ = oracle.hdbscan(BOUNDARY, 2, 1, eps)
+        assert same_partition(labels, np.repeat(np.arange(clusters), 8 // clusters))
+    labels, _ = oracle.hdbscan(np.zeros((8, 2)), 2)
+    assert same_partition(labels, np.full(8, -1))
+    for mcs, ms in ((2, 8), (8, 1), (9, 2)):
+        labels, _ = oracle.hdbscan(np.arange(16, dtype=np.float32).reshape(8, 2), mcs, ms)
+        assert np.array_equal(labels, np.full(8, -1))
+    # Documented float32 epsilon resolution limit.
+    eps = float(np.nextafter(np.float64(0.125), np.float64(1)))
+    labels, _ = oracle.hdbscan(BOUNDARY, 2, 1, eps, "float32")
+    assert same_partition(labels, [0, 0, 1, 1, 2, 2, 3, 3])
+    labels, _ = oracle.hdbscan(BOUNDARY.astype(np.float64), 2, 1, eps, "float64")
+    assert same_partition(labels, [0, 0, 0, 0, 1, 1, 1, 1])
+
+
+def test_status_flags():
+    x = np.zeros((3, 33, 1), np.float32)
+    x[-1, -1, 0] = np.finfo(np.float32).max
+    assert oracle.hdbscan(x, 2)[1] == 1
+    x[0, 0, 0] = np.finfo(np.float32).max
+    x[-1, -1, 0] = np.nan
+    assert oracle.hdbscan(x, 2)[1] & 2
