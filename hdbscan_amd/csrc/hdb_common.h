@@ -1,0 +1,75 @@
+// Shared device/host definitions of the gfx950 HDBSCAN kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hdb {
+
+#define HDB_HD __host__ __device__ __forceinline__
+
+// Largest LDS allocation of one workgroup on gfx950 (160 KiB per CU).
+constexpr uint32_t kLdsLimit = 160u * 1024u;
+
+template <typename Real> struct RealTraits;
+template <> struct RealTraits<float> {
+  HDB_HD static float max_value() { return 3.4028234663852886e+38F; }
+};
+template <> struct RealTraits<double> {
+  HDB_HD static double max_value() { return 1.7976931348623157e+308; }
+};
+
+HDB_HD uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
+HDB_HD size_t align256(size_t v) { return (v + 255u) & ~size_t(255u); }
+
+// Sort element: the Prim weight and the Prim step that produced it.  The
+// introsort compares weights only; the step index travels with it, so
+// sorting pairs performs exactly the swaps of NumPy's indirect sort.
+template <typename Real> struct alignas(2 * sizeof(Real) > 8 ? 16 : 8) EdgeKey {
+  Real w;
+  int32_t idx;
+};
+
+// Per-window tree arena.  Byte offsets from the window's arena base.
+// Lifetimes (arrays sharing storage are never live together):
+//   fixed   keys[n] (sort pairs; sorted weights feed condensation)
+//           edges[2n] int16: edge_u | edge_v, later the prune queue
+//           (2n entries), later label_map | resolved
+//   region  uf, comp (linkage) -> relabel, queue (condense) -> stop, mark (labels)
+//           left, right, sizes (linkage, condense) -> first_child,
+//           next_sibling, blocked (selection)
+//           parent, birth, stability, selected, point_parent
+//           sort stack
+struct TreeLayout {
+  uint32_t keys, edges, region;
+  uint32_t uf, comp, left, right, sizes, parent, birth, stab, selected, pparent, stack;
+  uint32_t total;
+};
+
+constexpr int kSortStack = 64;
+
+HDB_HD TreeLayout tree_layout(int n, int real_size) {
+  TreeLayout L;
+  uint32_t o = 0;
+  const uint32_t un = (uint32_t)n;
+  L.keys = o;
+  o = align16(o + (real_size == 4 ? 8u : 16u) * un);
+  L.edges = o;
+  o = align16(o + 4u * un);
+  L.region = o;
+  L.uf = o; o = align16(o + 2u * un);
+  L.comp = o; o = align16(o + 2u * un);
+  L.left = o; o = align16(o + 2u * un);
+  L.right = o; o = align16(o + 2u * un);
+  L.sizes = o; o = align16(o + 2u * un);
+  L.parent = o; o = align16(o + 2u * un);
+  L.birth = o; o = align16(o + (uint32_t)real_size * un);
+  L.stab = o; o = align16(o + (uint32_t)real_size * un);
+  L.selected = o; o = align16(o + un);
+  L.pparent = o; o = align16(o + 2u * un);
+  L.stack = o; o = align16(o + 3u * 4u * kSortStack);
+  L.total = o;
+  return L;
+}
+
+}  // namespace hdb

@@ -1,0 +1,297 @@
+// Distance, validation, core-distance and tree kernels.
+#pragma once
+
+#include "hdb_common.h"
+#include "hdb_tree.h"
+
+namespace hdb {
+
+// ORs HDB_STATUS_NONFINITE (2) into *status if any input value is not finite.
+template <typename Real>
+__global__ __launch_bounds__(256) void check_finite_kernel(const Real* x, size_t count, int32_t* status) {
+  int bad = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    bad |= !isfinite(x[i]);
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 2);
+}
+
+// Dense distances out[b][i][j] for a 64x64 tile per workgroup (4x4 per
+// thread).  Coordinates are staged through LDS in chunks of 16 dimensions;
+// each output sums (x_ik - x_jk)^2 for k ascending, then takes the correctly
+// rounded square root, so out[i][j] and out[j][i] are bitwise equal to the
+// reference's triangle-and-mirror kernel (reference hdbscan_gpu/_kernels.py:45-73).
+template <typename Real>
+__global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int d, Real* out, int32_t* status) {
+  constexpr int TILE = 64, KC = 16;
+  __shared__ Real si[KC][TILE];
+  __shared__ Real sj[KC][TILE];
+  const int b = blockIdx.z;
+  const int ti = blockIdx.y * TILE, tj = blockIdx.x * TILE;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const Real* xb = x + (size_t)b * n * d;
+  Real tot[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tot[r][q] = 0;
+  for (int k0 = 0; k0 < d; k0 += KC) {
+    __syncthreads();
+    for (int idx = tid; idx < TILE * KC; idx += 256) {
+      const int row = idx / KC, kk = idx - row * KC, k = k0 + kk;
+      const int gi = ti + row, gj = tj + row;
+      si[kk][row] = (gi < n && k < d) ? xb[(size_t)gi * d + k] : Real(0);
+      sj[kk][row] = (gj < n && k < d) ? xb[(size_t)gj * d + k] : Real(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) {
+      Real av[4], cv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) av[r] = si[kk][ty * 4 + r];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cv[q] = sj[kk][tx * 4 + q];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const Real dl = av[r] - cv[q];
+          tot[r][q] = tot[r][q] + dl * dl;
+        }
+    }
+  }
+  int bad = 0;
+  Real* ob = out + (size_t)b * n * n;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int gi = ti + ty * 4 + r;
+    if (gi >= n) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int gj = tj + tx * 4 + q;
+      if (gj < n) {
+        const Real v = sqrt(tot[r][q]);
+        bad |= !isfinite(v);
+        ob[(size_t)gi * n + gj] = v;
+      }
+    }
+  }
+  if (__any(bad) && (tid & 63) == 0) atomicOr(status, 1);
+}
+
+// Inserts v into the ascending register list lst, dropping the largest.
+template <typename Real, int KMAX>
+__device__ __forceinline__ void topk_insert(Real* lst, Real v) {
+  if (v < lst[KMAX - 1]) {
+#pragma unroll
+    for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
+    lst[0] = fmin(lst[0], v);
+  }
+}
+
+// Core distance of row i: the (ms-1)-th smallest of d(i, j) over all j
+// (self included), exactly as an order statistic of the reference's row
+// (reference hdbscan_gpu/_cuda.py:216-220).  One thread per row keeps the
+// KMAX smallest distances in a register insertion network while streaming
+// 64-point tiles of the window through LDS.  DK > 0: dimensions padded to DK
+// (zero padding adds +0 exactly); DK == 0: any d, 32-dimension chunks.
+template <typename Real, int KMAX, int DK>
+__global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, int d, int ms, Real* core) {
+  constexpr int TJ = DK > 0 ? 64 : 32;
+  constexpr int KC = DK > 0 ? DK : 32;
+  __shared__ Real tile[TJ * KC];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + tid;
+  const Real* xb = x + (size_t)b * n * d;
+  Real lst[KMAX];
+#pragma unroll
+  for (int q = 0; q < KMAX; ++q) lst[q] = Real(INFINITY);
+  Real xi[KC];
+  if (DK > 0) {
+#pragma unroll
+    for (int k = 0; k < KC; ++k) xi[k] = (i < n && k < d) ? xb[(size_t)i * d + k] : Real(0);
+  }
+  for (int j0 = 0; j0 < n; j0 += TJ) {
+    const int jn = min(TJ, n - j0);
+    if constexpr (DK > 0) {
+      __syncthreads();
+      for (int idx = tid; idx < TJ * KC; idx += blockDim.x) {
+        const int jj = idx / KC, k = idx - jj * KC, j = j0 + jj;
+        tile[idx] = (j < n && k < d) ? xb[(size_t)j * d + k] : Real(0);
+      }
+      __syncthreads();
+      for (int jj = 0; jj < jn; ++jj) {
+        Real t = 0;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          const Real dl = xi[k] - tile[jj * KC + k];
+          t = t + dl * dl;
+        }
+        topk_insert<Real, KMAX>(lst, sqrt(t));
+      }
+    } else {
+      Real tot[TJ];
+#pragma unroll
+      for (int jj = 0; jj < TJ; ++jj) tot[jj] = 0;
+      for (int k0 = 0; k0 < d; k0 += KC) {
+        __syncthreads();
+        for (int idx = tid; idx < TJ * KC; idx += blockDim.x) {
+          const int jj = idx / KC, kk = idx - jj * KC, j = j0 + jj, k = k0 + kk;
+          tile[idx] = (j < n && k < d) ? xb[(size_t)j * d + k] : Real(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KC; ++k) xi[k] = (i < n && k0 + k < d) ? xb[(size_t)i * d + k0 + k] : Real(0);
+#pragma unroll
+        for (int jj = 0; jj < TJ; ++jj) {
+          Real t = tot[jj];
+#pragma unroll
+          for (int k = 0; k < KC; ++k) {
+            const Real dl = xi[k] - tile[jj * KC + k];
+            t = t + dl * dl;
+          }
+          tot[jj] = t;
+        }
+      }
+      for (int jj = 0; jj < jn; ++jj) topk_insert<Real, KMAX>(lst, sqrt(tot[jj]));
+    }
+  }
+  if (i < n) {
+    Real v = lst[0];
+#pragma unroll
+    for (int q = 1; q < KMAX; ++q)
+      if (q == ms - 1) v = lst[q];
+    core[(size_t)b * n + i] = v;
+  }
+}
+
+template <typename Real> struct BitsOf;
+template <> struct BitsOf<float> {
+  using U = uint32_t;
+  __device__ static U get(float v) { return __float_as_uint(v); }
+  __device__ static float put(U u) { return __uint_as_float(u); }
+};
+template <> struct BitsOf<double> {
+  using U = uint64_t;
+  __device__ static U get(double v) { return (U)__double_as_longlong(v); }
+  __device__ static double put(U u) { return __longlong_as_double((long long)u); }
+};
+
+// Core distances for any min_samples from a materialised matrix: one wave per
+// row; the row is staged in LDS and the (ms-1)-th smallest value is built
+// bit by bit from the top (non-negative IEEE values order like their bits).
+template <typename Real>
+__global__ __launch_bounds__(64) void core_select_kernel(const Real* dist, int n, int ms, Real* core) {
+  using U = typename BitsOf<Real>::U;
+  extern __shared__ __align__(16) unsigned char lds[];
+  Real* row = reinterpret_cast<Real*>(lds);
+  const size_t r = blockIdx.x;
+  const Real* src = dist + r * n;
+  const int lane = threadIdx.x;
+  for (int j = lane; j < n; j += 64) row[j] = src[j];
+  __syncthreads();
+  const int k = ms - 1;
+  U ans = 0;
+  for (int bit = (int)sizeof(U) * 8 - 2; bit >= 0; --bit) {
+    const U cand = ans | ((U)1 << bit);
+    int cnt = 0;
+    for (int j = lane; j < n; j += 64) cnt += BitsOf<Real>::get(row[j]) < cand;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) cnt += __shfl_xor(cnt, m);
+    if (cnt <= k) ans = cand;
+  }
+  if (lane == 0) core[r] = BitsOf<Real>::put(ans);
+}
+
+struct TreeArgs {
+  const int32_t* edge_u;
+  const int32_t* edge_v;
+  const void* edge_w;
+  const int32_t* wflag;
+  int n;
+  long long mcs;
+  double epsilon;
+  int sort_depth;
+  int32_t* labels;
+  unsigned char* arena;  // global arena (when not in LDS)
+  size_t arena_stride;
+  int32_t* status;
+  int32_t* export_ints;  // [B][10][n] or null
+  void* export_reals;    // [B][4][n] or null
+};
+
+// Tree stages of one window per workgroup.  Lane 0 runs the sequential
+// stages; the workgroup stages inputs and writes labels and exports.
+template <typename Real, bool LDSA>
+__global__ __launch_bounds__(64) void tree_kernel(TreeArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int sh_next;
+  const int b = blockIdx.x, tid = threadIdx.x, T = blockDim.x, n = a.n;
+  int32_t* lab = a.labels + (size_t)b * n;
+  if (a.wflag[b]) {
+    for (int i = tid; i < n; i += T) lab[i] = -1;
+    return;
+  }
+  unsigned char* base = LDSA ? lds : a.arena + (size_t)b * a.arena_stride;
+  const TreeView<Real> t = tree_view<Real>(base, n);
+  const int32_t* eu = a.edge_u + (size_t)b * n;
+  const int32_t* ev = a.edge_v + (size_t)b * n;
+  const Real* ew = static_cast<const Real*>(a.edge_w) + (size_t)b * n;
+  for (int i = tid; i < n - 1; i += T) {
+    EdgeKey<Real> k;
+    k.w = ew[i];
+    k.idx = i;
+    t.keys[i] = k;
+    t.edge_u[i] = (int16_t)eu[i];
+    t.edge_v[i] = (int16_t)ev[i];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    introsort(t.keys, n - 1, a.sort_depth, t.stack);
+    linkage(t, n);
+    sh_next = condense(t, n, a.mcs);
+  }
+  __syncthreads();
+  const int nl = sh_next;
+  if (nl < 0) {
+    for (int i = tid; i < n; i += T) lab[i] = -1;
+    if (tid == 0) atomicOr(a.status, 1);
+    return;
+  }
+  int32_t* xi = a.export_ints ? a.export_ints + (size_t)b * 10 * n : nullptr;
+  Real* xr = a.export_reals ? static_cast<Real*>(a.export_reals) + (size_t)b * 4 * n : nullptr;
+  if (xi) {
+    for (int i = tid; i < n; i += T) {
+      const bool e = i < n - 1;
+      xi[0 * n + i] = e ? eu[i] : -1;
+      xi[1 * n + i] = e ? ev[i] : -1;
+      xi[2 * n + i] = e ? t.keys[i].idx : -1;
+      xi[3 * n + i] = e ? t.left[i] : -1;
+      xi[4 * n + i] = e ? t.right[i] : -1;
+      xi[5 * n + i] = e ? t.sizes[i] : -1;
+      xr[0 * n + i] = e ? ew[i] : Real(0);
+      xr[1 * n + i] = e ? t.keys[i].w : Real(0);
+      xr[2 * n + i] = i < nl - n ? t.birth[i] : Real(0);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) select_clusters(t, n, nl, (Real)a.epsilon);
+  __syncthreads();
+  if (xi) {
+    for (int i = tid; i < n; i += T) {
+      const bool c = i < nl - n;
+      xi[6 * n + i] = c ? t.parent[i] : -1;
+      xi[7 * n + i] = c ? t.selected[i] : 0;
+      xi[8 * n + i] = t.pparent[i];
+      xi[9 * n + i] = i == 0 ? nl : 0;
+      xr[3 * n + i] = c ? t.stab[i] : Real(0);
+    }
+  }
+  const int16_t* resolved = t.resolved();
+  for (int i = tid; i < n; i += T) {
+    const int pp = t.pparent[i];
+    lab[i] = pp >= n ? resolved[pp - n] : -1;
+  }
+}
+
+}  // namespace hdb

@@ -1,0 +1,229 @@
+// Cooperative Prim over the mutual-reachability graph of one window.
+//
+// One workgroup owns one window.  Thread t owns vertices t, t+T, t+2T, ...
+// (VPT slots) and keeps, per slot, the vertex's coordinates (points path),
+// core distance, best weight, best source and a visited bit in registers.
+// Each step:
+//   1. every thread forms the exact distance from the current vertex to its
+//      unvisited vertices (coordinate order, no FMA, correctly rounded sqrt;
+//      the current vertex is broadcast from LDS) -- or reads the row of a
+//      materialised distance matrix (matrix path),
+//   2. w = max(core_cur, core_j, d); strictly smaller w replaces best and
+//      source (equal weights keep the older source),
+//   3. the block-wide argmin of (best, vertex) -- lowest vertex on ties -- is
+//      one wave64 butterfly per wave plus one LDS exchange and ONE barrier.
+// Reference semantics: hdbscan_gpu/_kernels.py:164-320 (and sklearn
+// _linkage.pyx:111-224).  Output edges are (source, next, weight) in step
+// order, with source 0 when the weight is not below REAL_MAX
+// (reference hdbscan_gpu/_kernels.py:307-317).
+#pragma once
+
+#include "hdb_common.h"
+
+namespace hdb {
+
+// Candidate (weight, vertex) ordered lexicographically.  Weights are
+// non-negative or +inf, so their IEEE bit patterns order like the values.
+template <typename Real> struct Cand;
+
+template <> struct Cand<float> {
+  uint64_t k;
+  __device__ __forceinline__ static Cand make(float w, int j) {
+    Cand c;
+    c.k = ((uint64_t)__float_as_uint(w) << 32) | (uint32_t)j;
+    return c;
+  }
+  __device__ __forceinline__ bool less(const Cand& o) const { return k < o.k; }
+  __device__ __forceinline__ Cand xor_shuffle(int m) const {
+    Cand c;
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k, m);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k >> 32), m);
+    c.k = ((uint64_t)hi << 32) | lo;
+    return c;
+  }
+  __device__ __forceinline__ int index() const { return (int)(uint32_t)k; }
+  __device__ __forceinline__ float weight() const { return __uint_as_float((uint32_t)(k >> 32)); }
+};
+
+template <> struct Cand<double> {
+  uint64_t w;
+  uint32_t j;
+  __device__ __forceinline__ static Cand make(double v, int idx) {
+    Cand c;
+    c.w = (uint64_t)__double_as_longlong(v);
+    c.j = (uint32_t)idx;
+    return c;
+  }
+  __device__ __forceinline__ bool less(const Cand& o) const { return w < o.w || (w == o.w && j < o.j); }
+  __device__ __forceinline__ Cand xor_shuffle(int m) const {
+    Cand c;
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)w, m);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(w >> 32), m);
+    c.w = ((uint64_t)hi << 32) | lo;
+    c.j = (uint32_t)__shfl_xor((int)j, m);
+    return c;
+  }
+  __device__ __forceinline__ int index() const { return (int)j; }
+  __device__ __forceinline__ double weight() const { return __longlong_as_double((long long)w); }
+};
+
+template <typename Real>
+__device__ __forceinline__ Cand<Real> wave_min(Cand<Real> c) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    Cand<Real> o = c.xor_shuffle(m);
+    if (o.less(c)) c = o;
+  }
+  return c;
+}
+
+struct PrimArgs {
+  const void* x;      // points [B][n][d] (points path)
+  const void* dist;   // distances [B][n][n] (matrix path)
+  const void* core;   // [B][n] or null when unit_core
+  int n, d;
+  int unit_core;
+  int32_t* edge_u;    // [B][n]
+  int32_t* edge_v;    // [B][n]
+  void* edge_w;       // [B][n] Real
+  int32_t* wflag;     // [B] per-window error bits
+  int32_t* status;    // OR of all error bits
+};
+
+// MODE 0: points, coordinates of owned vertices in registers (DK padded dims,
+// zero padding adds +0 exactly), current vertex from LDS (XLDS) or memory.
+// MODE 1: matrix rows from device memory.
+template <typename Real, int VPT, int DK, int MODE, bool XLDS>
+__global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
+  using C = Cand<Real>;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
+  const int n = a.n, d = a.d;
+  const bool unit = a.unit_core != 0;
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ C red[2][16];
+  __shared__ int sh_err;
+  constexpr int DKS = DK > 0 ? DK : 1;
+  Real* xs = reinterpret_cast<Real*>(lds);
+  Real* cs = reinterpret_cast<Real*>(lds + (XLDS ? (size_t)n * DK * sizeof(Real) : 0));
+  const Real* xg = static_cast<const Real*>(a.x) + (size_t)b * n * (MODE == 0 ? d : 0);
+  const Real* dg = static_cast<const Real*>(a.dist) + (size_t)b * n * (MODE == 1 ? n : 0);
+  const Real* cg = static_cast<const Real*>(a.core) + (size_t)b * n;
+  if (tid == 0) sh_err = 0;
+  __syncthreads();
+  int err = 0;
+  if constexpr (MODE == 0 && XLDS) {
+    for (int idx = tid; idx < n * DKS; idx += T) {
+      const int i = idx / DKS, k = idx - i * DKS;
+      const Real v = k < d ? xg[(size_t)i * d + k] : Real(0);
+      if (!isfinite(v)) err |= 2;
+      xs[idx] = v;
+    }
+  }
+  if (!unit)
+    for (int i = tid; i < n; i += T) cs[i] = cg[i];
+
+  Real xr[VPT][DKS];
+  Real best[VPT], cr[VPT];
+  int src[VPT];
+  uint32_t vis = 0;
+#pragma unroll
+  for (int s = 0; s < VPT; ++s) {
+    const int j = tid + s * T;
+    best[s] = Real(INFINITY);
+    src[s] = 1;
+    cr[s] = 0;
+#pragma unroll
+    for (int k = 0; k < DKS; ++k) xr[s][k] = 0;
+    if (j < n) {
+      if (MODE == 0) {
+#pragma unroll
+        for (int k = 0; k < DKS; ++k) {
+          const Real v = k < d ? xg[(size_t)j * d + k] : Real(0);
+          if (!XLDS && !isfinite(v)) err |= 2;
+          xr[s][k] = v;
+        }
+      }
+      if (!unit) cr[s] = cg[j];
+    } else {
+      vis |= 1u << s;
+    }
+  }
+  if (tid == 0) vis |= 1u;  // vertex 0 starts the tree
+  if (err) atomicOr(&sh_err, err);
+  __syncthreads();
+
+  const Real rmax = RealTraits<Real>::max_value();
+  int cur = 0, par = 0, bad = 0;
+  for (int step = 0; step < n - 1; ++step) {
+    Real xc[DKS];
+    if (MODE == 0) {
+#pragma unroll
+      for (int k = 0; k < DKS; ++k)
+        xc[k] = XLDS ? xs[cur * DK + k] : (k < d ? xg[(size_t)cur * d + k] : Real(0));
+    }
+    const Real ccur = unit ? Real(0) : cs[cur];
+    C c = C::make(rmax, 0);
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      if (!((vis >> s) & 1u)) {
+        const int j = tid + s * T;
+        Real dd;
+        if (MODE == 0) {
+          Real tot = 0;
+#pragma unroll
+          for (int k = 0; k < DKS; ++k) {
+            const Real dl = xr[s][k] - xc[k];
+            tot = tot + dl * dl;
+          }
+          dd = sqrt(tot);
+        } else {
+          dd = dg[(size_t)cur * n + j];
+        }
+        bad |= !isfinite(dd);
+        Real w = dd;
+        if (!unit) w = fmax(fmax(ccur, cr[s]), dd);
+        if (w < best[s]) {
+          best[s] = w;
+          src[s] = cur;
+        }
+        const C o = C::make(best[s], j);
+        if (o.less(c)) c = o;
+      }
+    }
+    c = wave_min(c);
+    if (lane == 0) red[par][wave] = c;
+    __syncthreads();
+    C g = red[par][0];
+    for (int w = 1; w < W; ++w) {
+      const C o = red[par][w];
+      if (o.less(g)) g = o;
+    }
+    par ^= 1;
+    const int nxt = g.index();
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      if (nxt == tid + s * T) {
+        // A visited winner means no vertex had a weight below REAL_MAX: the
+        // graph is degenerate (only possible with non-finite weights).
+        if ((vis >> s) & 1u) bad |= 1;
+        vis |= 1u << s;
+        const Real gw = g.weight();
+        const size_t o = (size_t)b * n + step;
+        a.edge_u[o] = gw < rmax ? src[s] : 0;
+        a.edge_v[o] = nxt;
+        static_cast<Real*>(a.edge_w)[o] = gw;
+      }
+    }
+    cur = nxt;
+  }
+  if (bad) atomicOr(&sh_err, 1);
+  __syncthreads();
+  if (tid == 0) {
+    a.wflag[b] = sh_err;
+    if (sh_err) atomicOr(a.status, sh_err);
+  }
+}
+
+}  // namespace hdb

@@ -1,0 +1,414 @@
+// C ABI of the gfx950 batched HDBSCAN hot path (see include/hdbscan_rocm.h).
+//
+// Pipeline per call, all on the caller's stream:
+//   [core]  core_topk_kernel (min_samples 3..64, from points) or
+//           distance_kernel + core_select_kernel (min_samples > 64)
+//   [prim]  prim_kernel: points path (distances formed on the fly from
+//           register/LDS-resident coordinates) or matrix path (distance_kernel
+//           first, Prim reads rows)
+//   [tree]  tree_kernel: sort, linkage, condensation, selection, labels
+// Workspace (caller-owned device memory, hdb_workspace_bytes):
+//   edge_u[B][n] i32 | edge_v[B][n] i32 | edge_w[B][n] Real | wflag[B] i32 |
+//   core[B][n] Real | tree arena[B][stride] | distances[B][n][n] Real
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <unordered_set>
+
+#include "../../include/hdbscan_rocm.h"
+#include "hdb_common.h"
+#include "hdb_kernels.h"
+#include "hdb_prim.h"
+
+using namespace hdb;
+
+namespace {
+
+thread_local std::string g_error;
+
+// Optional per-stage HIP event timers of the calling thread (hdb_profile).
+struct StageTimers {
+  bool on = false;
+  int count = 0;
+  hipEvent_t events[16] = {};
+  const char* names[16] = {};
+};
+thread_local StageTimers g_timers;
+
+void stage_mark(hipStream_t s, const char* name) {
+  StageTimers& t = g_timers;
+  if (!t.on || t.count >= 16) return;
+  if (!t.events[t.count]) hipEventCreate(&t.events[t.count]);
+  hipEventRecord(t.events[t.count], s);
+  t.names[t.count] = name;
+  ++t.count;
+}
+
+int fail(const std::string& msg) {
+  g_error = msg;
+  return -1;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+// Raises a kernel's dynamic LDS limit once per kernel.
+int allow_lds(const void* fn, uint32_t bytes) {
+  static std::mutex mu;
+  static std::unordered_set<const void*> done;
+  if (bytes <= 64u * 1024u) return 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.count(fn)) return 0;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
+  if (e != hipSuccess) return fail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  done.insert(fn);
+  return 0;
+}
+
+struct Plan {
+  int rs = 4;             // sizeof(Real)
+  bool matrix = false;    // Prim reads a materialised matrix
+  int vpt = 1, dk = 0, threads = 64;
+  bool xlds = false;      // points path: all points in LDS
+  uint32_t prim_lds = 0;
+  bool core = false;      // core distances needed (min_samples > 2)
+  bool core_topk = false; // core from points with a register list
+  int kmax = 0, core_dk = 0;
+  bool tree_lds = true;
+  uint32_t tree_lds_bytes = 0;
+  size_t arena_stride = 0;
+  bool need_matrix = false;
+  size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_matrix = 0, total = 0;
+};
+
+int pad_dims(int64_t d) {
+  if (d <= 8) return 8;
+  if (d <= 16) return 16;
+  if (d <= 32) return 32;
+  if (d <= 64) return 64;
+  return 0;
+}
+
+int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int flags, Plan* p) {
+  if (dtype != HDB_F32 && dtype != HDB_F64) return fail("dtype must be HDB_F32 or HDB_F64");
+  if (batch < 1 || d < 1 || n < 2) return fail("X requires nonempty batches/features and at least 2 samples");
+  if (n > HDB_MAX_SAMPLES) return fail("at most 4096 samples per window are supported");
+  if (ms < 1 || ms > n) return fail("min_samples must be in [1, n]");
+  if (d > (1 << 24) || batch > (1 << 24)) return fail("dimension or batch too large");
+  Plan P;
+  P.rs = dtype == HDB_F32 ? 4 : 8;
+  const int rs = P.rs;
+  P.core = ms > 2;
+  P.core_topk = P.core && ms <= 64;
+  P.kmax = ms <= 8 ? 8 : ms <= 16 ? 16 : ms <= 32 ? 32 : 64;
+  P.core_dk = pad_dims(d);
+  // Points path: owned coordinates must fit 64 VGPRs per thread.
+  int want_vpt = (flags >> 8) & 15;
+  int vpt = want_vpt ? want_vpt : (n <= 1024 ? 1 : n <= 2048 ? 2 : 4);
+  if (vpt != 1 && vpt != 2 && vpt != 4) return fail("vertices per thread must be 1, 2 or 4");
+  while ((int64_t)vpt * 1024 < n) vpt *= 2;
+  if (vpt > 4) return fail("window too large for the vertex layout");
+  P.vpt = vpt;
+  P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
+  const int dk = pad_dims(d);
+  P.dk = dk;
+  P.matrix = (flags & HDB_FLAG_MATRIX) || dk == 0 || (int64_t)vpt * dk * rs > 64 * 4;
+  const uint32_t core_bytes = P.core ? (uint32_t)(n * rs) : 0u;
+  if (!P.matrix) {
+    const uint64_t xs = (uint64_t)n * dk * rs;
+    P.xlds = !(flags & HDB_FLAG_NO_POINT_LDS) && xs + core_bytes + 1024 <= kLdsLimit - 2048;
+    P.prim_lds = (uint32_t)((P.xlds ? xs : 0) + core_bytes);
+  } else {
+    P.prim_lds = core_bytes;
+  }
+  const TreeLayout L = tree_layout((int)n, rs);
+  P.tree_lds = !(flags & HDB_FLAG_GLOBAL_ARENA) && L.total + 1024 <= kLdsLimit;
+  P.tree_lds_bytes = P.tree_lds ? L.total : 0u;
+  P.arena_stride = P.tree_lds ? 0 : align256(L.total);
+  P.need_matrix = !(flags & HDB_FLAG_MATRIX_INPUT) && (P.matrix || (P.core && !P.core_topk));
+  size_t o = 0;
+  const size_t bn = (size_t)batch * n;
+  P.off_eu = o; o = align256(o + bn * 4);
+  P.off_ev = o; o = align256(o + bn * 4);
+  P.off_ew = o; o = align256(o + bn * rs);
+  P.off_flag = o; o = align256(o + (size_t)batch * 4);
+  P.off_core = o; o = align256(o + (P.core ? bn * rs : 0));
+  P.off_arena = o; o = align256(o + (size_t)batch * P.arena_stride);
+  P.off_matrix = o; o = align256(o + (P.need_matrix ? bn * (size_t)n * rs : 0));
+  P.total = o;
+  *p = P;
+  return 0;
+}
+
+template <typename Real>
+int launch_distances(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_t d, Real* out, int32_t* status) {
+  const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
+  distance_kernel<Real><<<grid, 256, 0, s>>>(x, (int)n, (int)d, out, status);
+  return check_launch("distance_kernel");
+}
+
+template <typename Real, int KMAX>
+int launch_topk_k(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_t d, int ms, Real* core, int dk) {
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)batch);
+  switch (dk) {
+    case 8: core_topk_kernel<Real, KMAX, 8><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
+    case 16: core_topk_kernel<Real, KMAX, 16><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
+    case 32: core_topk_kernel<Real, KMAX, 32><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
+    case 64: core_topk_kernel<Real, KMAX, 64><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
+    default: core_topk_kernel<Real, KMAX, 0><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
+  }
+  return check_launch("core_topk_kernel");
+}
+
+template <typename Real>
+int launch_core(hipStream_t s, const Plan& P, const Real* x, int64_t batch, int64_t n, int64_t d, int ms,
+                Real* core, Real* matrix, int32_t* status, bool matrix_ready) {
+  if (P.core_topk) {
+    switch (P.kmax) {
+      case 8: return launch_topk_k<Real, 8>(s, x, batch, n, d, ms, core, P.core_dk);
+      case 16: return launch_topk_k<Real, 16>(s, x, batch, n, d, ms, core, P.core_dk);
+      case 32: return launch_topk_k<Real, 32>(s, x, batch, n, d, ms, core, P.core_dk);
+      default: return launch_topk_k<Real, 64>(s, x, batch, n, d, ms, core, P.core_dk);
+    }
+  }
+  if (!matrix_ready) {
+    if (int r = launch_distances<Real>(s, x, batch, n, d, matrix, status)) return r;
+  }
+  const uint32_t bytes = (uint32_t)(n * sizeof(Real));
+  core_select_kernel<Real><<<(unsigned)(batch * n), 64, bytes, s>>>(matrix, (int)n, ms, core);
+  return check_launch("core_select_kernel");
+}
+
+template <typename Real, int VPT, int DK, int MODE, bool XLDS>
+int launch_prim_one(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
+  auto fn = prim_kernel<Real, VPT, DK, MODE, XLDS>;
+  if (int r = allow_lds((const void*)fn, P.prim_lds)) return r;
+  fn<<<(unsigned)batch, P.threads, P.prim_lds, s>>>(a);
+  return check_launch("prim_kernel");
+}
+
+template <typename Real, int VPT, int DK>
+int launch_prim_x(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
+  if constexpr (VPT * DK * sizeof(Real) <= 64 * 4) {
+    if (P.xlds) return launch_prim_one<Real, VPT, DK, 0, true>(s, P, batch, a);
+    return launch_prim_one<Real, VPT, DK, 0, false>(s, P, batch, a);
+  }
+  return fail("internal: unsupported points layout");
+}
+
+template <typename Real, int VPT>
+int launch_prim_v(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
+  if (P.matrix) return launch_prim_one<Real, VPT, 0, 1, false>(s, P, batch, a);
+  switch (P.dk) {
+    case 8: return launch_prim_x<Real, VPT, 8>(s, P, batch, a);
+    case 16: return launch_prim_x<Real, VPT, 16>(s, P, batch, a);
+    case 32: return launch_prim_x<Real, VPT, 32>(s, P, batch, a);
+    case 64: return launch_prim_x<Real, VPT, 64>(s, P, batch, a);
+  }
+  return fail("internal: unsupported dimension padding");
+}
+
+template <typename Real>
+int launch_prim(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
+  switch (P.vpt) {
+    case 1: return launch_prim_v<Real, 1>(s, P, batch, a);
+    case 2: return launch_prim_v<Real, 2>(s, P, batch, a);
+    case 4: return launch_prim_v<Real, 4>(s, P, batch, a);
+  }
+  return fail("internal: unsupported vertices per thread");
+}
+
+template <typename Real>
+int launch_tree(hipStream_t s, const Plan& P, int64_t batch, const TreeArgs& a) {
+  if (P.tree_lds) {
+    auto fn = tree_kernel<Real, true>;
+    if (int r = allow_lds((const void*)fn, P.tree_lds_bytes)) return r;
+    fn<<<(unsigned)batch, 64, P.tree_lds_bytes, s>>>(a);
+  } else {
+    tree_kernel<Real, false><<<(unsigned)batch, 64, 0, s>>>(a);
+  }
+  return check_launch("tree_kernel");
+}
+
+template <typename Real>
+int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_dist, const Real* given_core,
+                int64_t batch, int64_t n, int64_t d, long long mcs, int ms, bool unit, double eps,
+                int sort_depth, int32_t* labels, unsigned char* ws, int32_t* status, int32_t* exp_i, void* exp_r) {
+  int32_t* eu = reinterpret_cast<int32_t*>(ws + P.off_eu);
+  int32_t* ev = reinterpret_cast<int32_t*>(ws + P.off_ev);
+  Real* ew = reinterpret_cast<Real*>(ws + P.off_ew);
+  int32_t* wflag = reinterpret_cast<int32_t*>(ws + P.off_flag);
+  Real* core = reinterpret_cast<Real*>(ws + P.off_core);
+  Real* matrix = reinterpret_cast<Real*>(ws + P.off_matrix);
+  const Real* dist = given_dist;
+  const Real* core_in = given_core;
+  g_timers.count = 0;
+  stage_mark(s, "start");
+  if (!given_dist) {
+    bool matrix_ready = false;
+    if (P.matrix) {
+      if (int r = launch_distances<Real>(s, x, batch, n, d, matrix, status)) return r;
+      matrix_ready = true;
+      dist = matrix;
+      stage_mark(s, "distance_kernel");
+    }
+    if (!unit) {
+      if (int r = launch_core<Real>(s, P, x, batch, n, d, ms, core, matrix, status, matrix_ready)) return r;
+      core_in = core;
+      stage_mark(s, P.core_topk ? "core_topk_kernel" : "core_select_kernel");
+    }
+  }
+  PrimArgs pa;
+  pa.x = x;
+  pa.dist = dist;
+  pa.core = unit ? nullptr : core_in;
+  pa.n = (int)n;
+  pa.d = (int)d;
+  pa.unit_core = unit ? 1 : 0;
+  pa.edge_u = eu;
+  pa.edge_v = ev;
+  pa.edge_w = ew;
+  pa.wflag = wflag;
+  pa.status = status;
+  if (int r = launch_prim<Real>(s, P, batch, pa)) return r;
+  stage_mark(s, "prim_kernel");
+  TreeArgs ta;
+  ta.edge_u = eu;
+  ta.edge_v = ev;
+  ta.edge_w = ew;
+  ta.wflag = wflag;
+  ta.n = (int)n;
+  ta.mcs = mcs;
+  ta.epsilon = eps;
+  ta.sort_depth = sort_depth;
+  ta.labels = labels;
+  ta.arena = ws + P.off_arena;
+  ta.arena_stride = P.arena_stride;
+  ta.status = status;
+  ta.export_ints = exp_i;
+  ta.export_reals = exp_r;
+  if (int r = launch_tree<Real>(s, P, batch, ta)) return r;
+  stage_mark(s, "tree_kernel");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdb_version(void) { return 100; }
+
+int hdb_profile(int enable) {
+  g_timers.on = enable != 0;
+  g_timers.count = 0;
+  return 0;
+}
+
+int hdb_stage_count(void) { return g_timers.count > 0 ? g_timers.count - 1 : 0; }
+
+const char* hdb_stage_name(int i) {
+  if (i < 0 || i + 1 >= g_timers.count) return "";
+  return g_timers.names[i + 1];
+}
+
+float hdb_stage_ms(int i) {
+  if (i < 0 || i + 1 >= g_timers.count) return -1.0f;
+  float ms = -1.0f;
+  if (hipEventElapsedTime(&ms, g_timers.events[i], g_timers.events[i + 1]) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+const char* hdb_last_error(void) { return g_error.c_str(); }
+
+size_t hdb_workspace_bytes(int64_t batch, int64_t n, int64_t d, int dtype, int32_t min_samples, int flags) {
+  Plan P;
+  if (make_plan(batch, n, d, dtype, min_samples, flags, &P)) return 0;
+  return P.total;
+}
+
+int hdb_cluster(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,
+                int64_t min_cluster_size, int32_t min_samples, double epsilon, int32_t* labels,
+                void* workspace, size_t workspace_bytes, int32_t* status, int flags) {
+  g_error.clear();
+  if (!x || !labels || !status) return fail("null pointer argument");
+  if (min_cluster_size < 2) return fail("min_cluster_size must be >= 2");
+  Plan P;
+  if (make_plan(batch, n, d, dtype, min_samples, flags, &P)) return -1;
+  if (workspace_bytes < P.total || (P.total && !workspace)) return fail("workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
+  const bool unit = min_samples <= 2;
+  if (dtype == HDB_F32)
+    return run_cluster<float>(s, P, static_cast<const float*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+                              min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
+  return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+                             min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
+}
+
+int hdb_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype, void* out,
+                  int32_t* status) {
+  g_error.clear();
+  if (!x || !out || !status) return fail("null pointer argument");
+  Plan P;
+  if (make_plan(batch, n, d, dtype, 1, 0, &P)) return -1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t count = (size_t)batch * n * d;
+  const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 4096);
+  if (dtype == HDB_F32) {
+    check_finite_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float*>(x), count, status);
+    if (int r = check_launch("check_finite_kernel")) return r;
+    return launch_distances<float>(s, static_cast<const float*>(x), batch, n, d, static_cast<float*>(out), status);
+  }
+  check_finite_kernel<double><<<blocks, 256, 0, s>>>(static_cast<const double*>(x), count, status);
+  if (int r = check_launch("check_finite_kernel")) return r;
+  return launch_distances<double>(s, static_cast<const double*>(x), batch, n, d, static_cast<double*>(out), status);
+}
+
+int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,
+                       int32_t min_samples, void* core, void* workspace, size_t workspace_bytes,
+                       int32_t* status) {
+  g_error.clear();
+  if (!x || !core || !status) return fail("null pointer argument");
+  Plan P;
+  if (make_plan(batch, n, d, dtype, min_samples, 0, &P)) return -1;
+  P.core = true;
+  P.core_topk = min_samples <= 64;
+  P.kmax = min_samples <= 8 ? 8 : min_samples <= 16 ? 16 : min_samples <= 32 ? 32 : 64;
+  const size_t need = P.core_topk ? 0 : (size_t)batch * n * n * P.rs;
+  if (workspace_bytes < need || (need && !workspace)) return fail("workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == HDB_F32)
+    return launch_core<float>(s, P, static_cast<const float*>(x), batch, n, d, min_samples, static_cast<float*>(core),
+                              static_cast<float*>(workspace), status, false);
+  return launch_core<double>(s, P, static_cast<const double*>(x), batch, n, d, min_samples,
+                             static_cast<double*>(core), static_cast<double*>(workspace), status, false);
+}
+
+int hdb_cluster_matrix(void* stream, const void* distances, const void* core, int64_t batch, int64_t n, int dtype,
+                       int64_t min_cluster_size, int unit_core, double epsilon, int sort_depth, int32_t* labels,
+                       void* workspace, size_t workspace_bytes, int32_t* status, int flags, int32_t* export_ints,
+                       void* export_reals) {
+  g_error.clear();
+  if (!distances || !labels || !status) return fail("null pointer argument");
+  if (!unit_core && !core) return fail("core distances required unless unit_core");
+  if (min_cluster_size < 2) return fail("min_cluster_size must be >= 2");
+  if ((export_ints == nullptr) != (export_reals == nullptr)) return fail("export buffers must be given together");
+  Plan P;
+  if (make_plan(batch, n, 1, dtype, unit_core ? 1 : 3, flags | HDB_FLAG_MATRIX | HDB_FLAG_MATRIX_INPUT, &P)) return -1;
+  if (workspace_bytes < P.total || !workspace) return fail("workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
+  if (dtype == HDB_F32)
+    return run_cluster<float>(s, P, nullptr, static_cast<const float*>(distances), static_cast<const float*>(core),
+                              batch, n, 1, min_cluster_size, unit_core ? 1 : 3, unit_core != 0, epsilon, sort_depth,
+                              labels, ws, status, export_ints, export_reals);
+  return run_cluster<double>(s, P, nullptr, static_cast<const double*>(distances), static_cast<const double*>(core),
+                             batch, n, 1, min_cluster_size, unit_core ? 1 : 3, unit_core != 0, epsilon, sort_depth,
+                             labels, ws, status, export_ints, export_reals);
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+/*
+ * hdbscan_rocm -- C ABI of the MI355X (gfx950) batched HDBSCAN hot path.
+ *
+ * Every pointer is a device pointer owned by the caller; the library
+ * allocates no memory and enqueues work only on the given HIP stream
+ * (hipStream_t passed as void*; NULL = the null stream).  Calls on different
+ * streams may overlap; calls that share a workspace must not.
+ *
+ * Precision codes: HDB_F32 (float) and HDB_F64 (double).  x is a dense
+ * row-major [batch][n][d] array of that type.
+ *
+ * Status: *status (device int32) is OR-accumulated; the caller zeroes it
+ * before the call and reads it after synchronising.  HDB_STATUS_NONFINITE
+ * outranks HDB_STATUS_OVERFLOW when reporting.  Labels of a call whose
+ * status is non-zero are undefined.
+ *
+ * Return value: 0 on success, otherwise a negative code; hdb_last_error()
+ * then describes the failure (invalid argument or HIP launch error).
+ *
+ * The functions replace the reference's CUDA backend entry
+ *   cluster(x, min_cluster_size, min_samples, epsilon, precision)
+ *   (reference hdbscan_gpu/_cuda.py:193-230, called from
+ *    hdbscan_gpu/_api.py:92-102)
+ * and its kernel launches K1/K2 and the library k-selection
+ *   (reference hdbscan_gpu/_cuda.py:208-229).
+ */
+#ifndef HDBSCAN_ROCM_H
+#define HDBSCAN_ROCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDB_F32 0
+#define HDB_F64 1
+
+#define HDB_MAX_SAMPLES 4096
+
+#define HDB_STATUS_OVERFLOW 1  /* a pairwise distance is not finite */
+#define HDB_STATUS_NONFINITE 2 /* an input coordinate is not finite */
+
+/* Flags for hdb_cluster / hdb_workspace_bytes (tuning and test knobs). */
+#define HDB_FLAG_GLOBAL_ARENA 1 /* tree arena in device memory, not LDS */
+#define HDB_FLAG_MATRIX 2       /* materialise distances, Prim reads rows */
+#define HDB_FLAG_NO_POINT_LDS 4 /* current vertex read from device memory */
+#define HDB_FLAG_MATRIX_INPUT 8 /* sizing for hdb_cluster_matrix (distances given) */
+/* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic. */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int hdb_version(void);
+
+/* Text of the last error of the calling thread ("" if none). */
+const char *hdb_last_error(void);
+
+/* Stage timers of the calling thread (no counterpart in the reference; the
+ * reference times stages from Python, scripts/profile_cuda.py:49-126).
+ * With hdb_profile(1), every hdb_cluster / hdb_cluster_matrix call records a
+ * HIP event after each kernel stage on its stream.  After synchronising,
+ * hdb_stage_ms(i) is the duration of stage i of the last call on this thread
+ * and hdb_stage_name(i) its kernel name. */
+int hdb_profile(int enable);
+int hdb_stage_count(void);
+const char *hdb_stage_name(int i);
+float hdb_stage_ms(int i);
+
+/* Bytes of device workspace hdb_cluster needs for these arguments.
+ * Replaces the reference's global-workspace sizing
+ * (reference hdbscan_gpu/_cuda.py:164-165, 221-223). */
+size_t hdb_workspace_bytes(int64_t batch, int64_t n, int64_t d, int dtype,
+                           int32_t min_samples, int flags);
+
+/* Labels-only Euclidean EOM HDBSCAN of `batch` independent windows.
+ * labels: [batch][n] int32, -1 = noise.  min_samples is self-inclusive;
+ * epsilon is converted to the working precision.
+ * Replaces reference hdbscan_gpu/_cuda.py:193-230 (cluster). */
+int hdb_cluster(void *stream, const void *x, int64_t batch, int64_t n, int64_t d,
+                int dtype, int64_t min_cluster_size, int32_t min_samples,
+                double epsilon, int32_t *labels, void *workspace,
+                size_t workspace_bytes, int32_t *status, int flags);
+
+/* Dense pairwise distances out[batch][n][n] (diagonal +0), bitwise equal to
+ * the reference's triangle-and-mirror kernel.
+ * Replaces reference hdbscan_gpu/_kernels.py:45-88 (distances_kernel). */
+int hdb_distances(void *stream, const void *x, int64_t batch, int64_t n,
+                  int64_t d, int dtype, void *out, int32_t *status);
+
+/* Core distances core[batch][n]: the (min_samples-1)-th smallest distance of
+ * each row, self included.  Needs hdb_workspace_bytes(..., HDB_FLAG_MATRIX)
+ * bytes of workspace when min_samples > 64 (0 otherwise).
+ * Replaces reference hdbscan_gpu/_cuda.py:216-220 (cp.partition). */
+int hdb_core_distances(void *stream, const void *x, int64_t batch, int64_t n,
+                       int64_t d, int dtype, int32_t min_samples, void *core,
+                       void *workspace, size_t workspace_bytes, int32_t *status);
+
+/* Graph and tree stages from a given distance matrix [batch][n][n] and core
+ * distances [batch][n] (core may be NULL when unit_core != 0, meaning every
+ * mutual-reachability weight equals the distance).  When export_ints /
+ * export_reals are non-NULL every intermediate is written per window:
+ *   export_ints  [batch][10][n] int32: edge_u, edge_v, order, left, right,
+ *                sizes, parent, selected, point_parent, {next_label, ...}
+ *   export_reals [batch][4][n] Real:  edge_w, distance, birth, stability
+ * (cluster-indexed arrays hold cluster id - n).  sort_depth < -1000 keeps
+ * NumPy's depth; other values force the introsort depth.
+ * Replaces reference hdbscan_gpu/_cuda.py:224-229 (clusters_kernel launch)
+ * and the diagnostic workspace export of reference tests/test_kernels.py:197-266. */
+int hdb_cluster_matrix(void *stream, const void *distances, const void *core,
+                       int64_t batch, int64_t n, int dtype, int64_t min_cluster_size,
+                       int unit_core, double epsilon, int sort_depth,
+                       int32_t *labels, void *workspace, size_t workspace_bytes,
+                       int32_t *status, int flags, int32_t *export_ints,
+                       void *export_reals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDBSCAN_ROCM_H */

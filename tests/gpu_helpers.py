@@ -1,0 +1,146 @@
+"""Calls into libhdbscan_rocm.so through its C ABI for the GPU parity tests."""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from hdbscan_amd import _native
+
+P = ctypes.c_void_p
+
+
+def torch():
+    import torch as t
+
+    return t
+
+
+def dev(a, dtype=None):
+    """Host array -> device tensor through a freshly allocated pinned buffer
+    (pageable copies of large arrays are not available on every host)."""
+    t = torch()
+    arr = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
+    pinned = t.from_numpy(arr).pin_memory()
+    out = pinned.to("cuda", non_blocking=True)
+    t.cuda.synchronize()
+    return out
+
+
+def host(x):
+    t = torch()
+    t.cuda.synchronize()
+    pinned = t.empty(x.shape, dtype=x.dtype, pin_memory=True)
+    pinned.copy_(x)
+    t.cuda.synchronize()
+    return pinned.numpy().copy()
+
+
+def ptr(t):
+    return None if t is None else P(t.data_ptr())
+
+
+def stream():
+    return P(torch().cuda.current_stream().cuda_stream)
+
+
+def code(precision):
+    return _native.F32 if precision == "float32" else _native.F64
+
+
+def real(precision):
+    return np.float32 if precision == "float32" else np.float64
+
+
+def distances(x, precision):
+    t = torch()
+    x = np.asarray(x, real(precision))
+    if x.ndim == 2:
+        x = x[None]
+    b, n, d = x.shape
+    xd = dev(x)
+    out = t.empty((b, n, n), dtype=xd.dtype, device="cuda")
+    status = t.zeros(1, dtype=t.int32, device="cuda")
+    _native.check(_native.lib().hdb_distances(stream(), ptr(xd), b, n, d, code(precision), ptr(out), ptr(status)))
+    return host(out), int(host(status)[0])
+
+
+def core(x, min_samples, precision):
+    t = torch()
+    x = np.asarray(x, real(precision))
+    if x.ndim == 2:
+        x = x[None]
+    b, n, d = x.shape
+    xd = dev(x)
+    out = t.empty((b, n), dtype=xd.dtype, device="cuda")
+    status = t.zeros(1, dtype=t.int32, device="cuda")
+    ws_bytes = b * n * n * x.itemsize
+    ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
+    _native.check(_native.lib().hdb_core_distances(stream(), ptr(xd), b, n, d, code(precision), int(min_samples),
+                                                   ptr(out), ptr(ws), ws_bytes, ptr(status)))
+    return host(out), int(host(status)[0])
+
+
+def cluster(x, mcs, ms, eps, precision, flags=0):
+    """Labels [B, N] (numpy) and status from hdb_cluster."""
+    t = torch()
+    x = np.asarray(x, real(precision))
+    squeeze = x.ndim == 2
+    if squeeze:
+        x = x[None]
+    b, n, d = x.shape
+    lib = _native.lib()
+    ws_bytes = lib.hdb_workspace_bytes(b, n, d, code(precision), int(ms), int(flags))
+    _native.check(0 if ws_bytes else -1)
+    xd = dev(x)
+    ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
+    labels = t.full((b, n), -7, dtype=t.int32, device="cuda")
+    status = t.zeros(1, dtype=t.int32, device="cuda")
+    _native.check(lib.hdb_cluster(stream(), ptr(xd), b, n, d, code(precision), int(mcs), int(ms), float(eps),
+                                  ptr(labels), ptr(ws), ws_bytes, ptr(status), int(flags)))
+    out = host(labels)
+    return (out[0] if squeeze else out), int(host(status)[0])
+
+
+EXPORT_INTS = ("edge_u", "edge_v", "order", "left", "right", "sizes", "parent", "selected", "point_parent", "meta")
+EXPORT_REALS = ("edge_w", "distance", "birth", "stability")
+
+
+def cluster_matrix(dist, core_dist, mcs, eps, precision, depth=None, flags=0, export=True):
+    """Graph + tree stages from matrices; returns (labels, status, exports per window)."""
+    t = torch()
+    dist = np.asarray(dist, real(precision))
+    if dist.ndim == 2:
+        dist = dist[None]
+    b, n, _ = dist.shape
+    unit = core_dist is None
+    lib = _native.lib()
+    ws_bytes = lib.hdb_workspace_bytes(b, n, 1, code(precision), 1 if unit else 3,
+                                       int(flags) | _native.FLAG_MATRIX | _native.FLAG_MATRIX_INPUT)
+    _native.check(0 if ws_bytes else -1)
+    dd = dev(dist)
+    cd = None if unit else dev(np.asarray(core_dist, real(precision)).reshape(b, n))
+    ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
+    labels = t.full((b, n), -7, dtype=t.int32, device="cuda")
+    status = t.zeros(1, dtype=t.int32, device="cuda")
+    ei = t.zeros((b, 10, n), dtype=t.int32, device="cuda") if export else None
+    er = t.zeros((b, 4, n), dtype=dd.dtype, device="cuda") if export else None
+    _native.check(lib.hdb_cluster_matrix(stream(), ptr(dd), ptr(cd), b, n, code(precision), int(mcs), int(unit),
+                                         float(eps), -100000 if depth is None else int(depth), ptr(labels),
+                                         ptr(ws), ws_bytes, ptr(status), int(flags), ptr(ei), ptr(er)))
+    out = host(labels)
+    exports = []
+    if export:
+        hi, hr = host(ei), host(er)
+        for w in range(b):
+            e = {k: hi[w, i] for i, k in enumerate(EXPORT_INTS)}
+            e.update({k: hr[w, i] for i, k in enumerate(EXPORT_REALS)})
+            nl = int(e["meta"][0])
+            e["next_label"] = nl
+            for k in ("edge_u", "edge_v", "order", "left", "right", "sizes", "edge_w", "distance"):
+                e[k] = e[k][: n - 1]
+            for k in ("parent", "selected", "birth", "stability"):
+                e[k] = e[k][: nl - n]
+            exports.append(e)
+    return out, int(host(status)[0]), exports

@@ -1,0 +1,215 @@
+"""GPU parity: libhdbscan_rocm.so against the CPU oracle (bitwise) and the
+golden scikit-learn fixtures.  Every call goes through the C ABI."""
+
+import numpy as np
+import pytest
+
+import gpu_helpers as G
+from golden_io import records, same_partition, workload_windows
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+PRECISIONS = ["float32", "float64"]
+BITS = {"float32": np.uint32, "float64": np.uint64}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm GPU")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("size", [2, 33, 129, 1024])
+@pytest.mark.parametrize("dims", [1, 8, 31])
+def test_distances_bitwise(precision, size, dims):
+    rng = np.random.default_rng(size + dims)
+    x = rng.normal(size=(3, size, dims)).astype(precision)
+    x[0, 0] = 0
+    x[0, -1] = -0.0
+    x[1, -1] = x[1, 0]
+    x[2] *= 1e9
+    if size > 4:
+        x[1, 3] = x[1, 4] + np.asarray(1e-30, precision)
+    got, status = G.distances(x, precision)
+    assert status == 0
+    for b in range(3):
+        exp = oracle.distances(x[b], precision)
+        np.testing.assert_array_equal(got[b].view(BITS[precision]), exp.view(BITS[precision]))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("dims", [3, 16, 40, 100])
+@pytest.mark.parametrize("ms", [3, 5, 8, 17, 64, 65, 100])
+def test_core_distances_bitwise(precision, dims, ms):
+    rng = np.random.default_rng(ms * 7 + dims)
+    n = 300
+    x = rng.integers(-6, 7, (2, n, dims)).astype(precision) / 4
+    x[1] = rng.normal(size=(n, dims))
+    got, status = G.core(x, ms, precision)
+    assert status == 0
+    for b in range(2):
+        exp = oracle.core(oracle.distances(x[b], precision), ms, precision)
+        np.testing.assert_array_equal(got[b].view(BITS[precision]), exp.view(BITS[precision]))
+
+
+def path_graph(weights, dtype):
+    size = len(weights) + 1
+    dist = np.full((size, size), size * 10, dtype=dtype)
+    np.fill_diagonal(dist, 0)
+    for i, w in enumerate(weights):
+        dist[i, i + 1] = dist[i + 1, i] = w
+    return dist
+
+
+ARGSORT = records("argsort")
+
+
+@pytest.mark.parametrize("index", range(len(ARGSORT)), ids=[r["name"] for r in ARGSORT])
+def test_introsort_order_matches_numpy(index):
+    rec = ARGSORT[index]
+    precision = "float32" if rec["is32"] else "float64"
+    w = rec["w64"].astype(precision)
+    # Distances are never negative: signed fixtures are folded to |w| and
+    # checked against the (separately pinned) oracle order instead.
+    signed = bool((w < 0).any())
+    w = np.abs(w)
+    dist = path_graph(w, precision)
+    _, status, ex = G.cluster_matrix(dist, None, 2, 0.0, precision)
+    assert status == 0
+    np.testing.assert_array_equal(ex[0]["edge_w"], w)
+    expected = oracle.argsort(w) if signed else rec["order"]
+    np.testing.assert_array_equal(ex[0]["order"], expected)
+
+
+@pytest.mark.parametrize("size", [33, 129, 1024])
+@pytest.mark.parametrize("depth", [-1, 0, 1, 3])
+def test_forced_sort_depth_matches_oracle(size, depth):
+    rng = np.random.default_rng(size)
+    w = rng.integers(1, 4, size - 1).astype(np.float32)
+    dist = path_graph(w, np.float32)
+    _, status, ex = G.cluster_matrix(dist, None, 2, 0.0, "float32", depth=depth)
+    assert status == 0
+    np.testing.assert_array_equal(ex[0]["order"], oracle.argsort(w, depth=depth))
+
+
+def prim_matrices(size, dtype):
+    rng = np.random.default_rng(size)
+    equal = np.ones((size, size), dtype)
+    retained = np.full((size, size), 9, dtype)
+    if size >= 4:
+        for a, b, value in [(0, 1, 1), (0, 2, 2), (1, 3, 3), (2, 3, 3)]:
+            retained[a, b] = retained[b, a] = value
+    quantized = rng.integers(1, 7, size=(size, size)).astype(dtype)
+    quantized = np.minimum(quantized, quantized.T)
+    mats = np.stack([equal, retained, quantized])
+    for m in mats:
+        np.fill_diagonal(m, 0)
+    core = np.zeros((3, size), dtype)
+    core[2] = rng.integers(0, 4, size=size)
+    return mats, core
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("size", [2, 255, 256, 257, 513, 1500, 4096])
+@pytest.mark.parametrize("vpt", [0, 1, 2, 4])
+def test_matrix_pipeline_matches_oracle(precision, size, vpt):
+    mats, core = prim_matrices(size, precision)
+    labels, status, ex = G.cluster_matrix(mats, core, 2, 0.0, precision, flags=vpt << 8)
+    assert status == 0
+    for b in range(3):
+        u, v, w = oracle.prim(mats[b], core[b], precision=precision)
+        np.testing.assert_array_equal(ex[b]["edge_u"], u)
+        np.testing.assert_array_equal(ex[b]["edge_v"], v)
+        np.testing.assert_array_equal(ex[b]["edge_w"], w)
+        t = oracle.tree(u, v, w, 2, 0.0, precision=precision)
+        for k in ("order", "left", "right", "sizes", "parent", "selected", "point_parent"):
+            np.testing.assert_array_equal(ex[b][k], t[k], err_msg=k)
+        for k in ("distance", "birth", "stability"):
+            np.testing.assert_array_equal(ex[b][k].view(BITS[precision]), t[k].view(BITS[precision]), err_msg=k)
+        np.testing.assert_array_equal(labels[b], t["labels"])
+
+
+@pytest.mark.parametrize("size", [32, 128])
+@pytest.mark.parametrize("within_pair", [0.0, 0.25])
+@pytest.mark.parametrize("epsilon", [0.0, 0.1, 2.0])
+@pytest.mark.parametrize("minimum", [2, 5, 2**31 - 1])
+@pytest.mark.parametrize("arena", [0, 1])
+def test_eom_fixtures_match_oracle(size, within_pair, epsilon, minimum, arena):
+    groups = np.arange(size) // 2
+    chain = (np.maximum(groups[:, None], groups[None, :]) + 1).astype(np.float32)
+    chain[groups[:, None] == groups[None, :]] = within_pair
+    equal = np.ones_like(chain)
+    nested = np.full_like(chain, 10)
+    for width, distance in ((8, 1), (4, 100 / 101), (2, 10 / 11)):
+        membership = np.arange(size) // width
+        nested[membership[:, None] == membership[None, :]] = distance
+    mats = np.stack([chain, equal, nested])
+    for m in mats:
+        np.fill_diagonal(m, 0)
+    labels, status, ex = G.cluster_matrix(mats, np.zeros((3, size), np.float32), minimum, epsilon, "float32",
+                                          flags=arena)
+    assert status == 0
+    for b in range(3):
+        u, v, w = oracle.prim(mats[b], np.zeros(size, np.float32))
+        t = oracle.tree(u, v, w, minimum, epsilon)
+        for k in ("parent", "selected", "point_parent"):
+            np.testing.assert_array_equal(ex[b][k], t[k], err_msg=k)
+        for k in ("birth", "stability"):
+            np.testing.assert_array_equal(ex[b][k], t[k], err_msg=k)
+        np.testing.assert_array_equal(labels[b], t["labels"])
+
+
+CORPUS = records("corpus")
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_corpus_labels_match_oracle(precision):
+    mismatches = []
+    for r in CORPUS:
+        x = r["x"].astype(precision)
+        got, status = G.cluster(x, r["mcs"], r["ms"], r["eps"], precision)
+        exp, est = oracle.hdbscan(x, r["mcs"], r["ms"], r["eps"], precision=precision)
+        assert status == est == 0, r["name"]
+        if not np.array_equal(got, exp):
+            mismatches.append(r["name"])
+    assert not mismatches, mismatches[:20]
+
+
+WORKLOADS = workload_windows()
+
+
+@pytest.mark.parametrize("index", range(len(WORKLOADS)), ids=[w[0] for w in WORKLOADS])
+def test_workload_windows(index):
+    name, x, precision, mcs, ms, eps, sk_labels = WORKLOADS[index]
+    got, status = G.cluster(x, mcs, ms, eps, precision)
+    assert status == 0
+    exp, _ = oracle.hdbscan(x, mcs, ms, eps, precision=precision)
+    np.testing.assert_array_equal(got, exp)
+    assert same_partition(got, sk_labels)
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 4, 2 | 1, 1 << 8, 2 << 8, 4 << 8])
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_path_variants_agree(flags, precision):
+    rng = np.random.default_rng(11)
+    x = (rng.integers(-20, 21, (4, 700, 6)) / 8).astype(precision)
+    for mcs, ms, eps in ((2, 2, 0.1), (5, 5, 0.0), (7, 3, 0.05)):
+        got, status = G.cluster(x, mcs, ms, eps, precision, flags=flags)
+        assert status == 0
+        for b in range(4):
+            exp, _ = oracle.hdbscan(x[b], mcs, ms, eps, precision=precision)
+            np.testing.assert_array_equal(got[b], exp)
+
+
+def test_status_flags():
+    x = np.zeros((3, 33, 1), np.float32)
+    x[-1, -1, 0] = np.finfo(np.float32).max
+    assert G.cluster(x, 2, 2, 0.0, "float32")[1] == 1
+    x[0, 0, 0] = np.finfo(np.float32).max
+    x[-1, -1, 0] = np.nan
+    assert G.cluster(x, 2, 2, 0.0, "float32")[1] & 2
+    big = np.finfo(np.float64).max / 4
+    assert G.cluster(np.array([[-big], [-big], [big], [big]]), 2, 5 - 3, 0.0, "float64")[1] == 1
