@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-sklearn", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (3 launches) and exit; "
+                         "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one GPU, so --gpus {args.gpus} is not run. "
@@ -148,6 +151,15 @@ def main():
     from hdbscan_amd import _native, _rocm, hdbscan, workloads
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    if args.copy_calibration:
+        count = args.copy_calibration * 1024 * 1024 // 4
+        src = torch.ones(count, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": count * 4, "launches": 3}), flush=True)
+        return
     spec = dict(workloads.WORKLOADS[args.workload])
     if spec["n"] is None:
         raise SystemExit("ragged workloads are not a bench line")

@@ -10,7 +10,7 @@ LIBRARY = os.path.join(HERE, "_lib", "libhdbscan_rocm.so")
 
 F32, F64 = 0, 1
 STATUS_OVERFLOW, STATUS_NONFINITE = 1, 2
-FLAG_GLOBAL_ARENA, FLAG_MATRIX, FLAG_NO_POINT_LDS, FLAG_MATRIX_INPUT = 1, 2, 4, 8
+FLAG_GLOBAL_ARENA, FLAG_MATRIX, FLAG_NO_POINT_LDS, FLAG_MATRIX_INPUT, FLAG_SERIAL_SORT = 1, 2, 4, 8, 16
 EXPORTED = ("hdb_version", "hdb_last_error", "hdb_profile", "hdb_stage_count", "hdb_stage_name",
             "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_distances",
             "hdb_core_distances", "hdb_cluster_matrix")

@@ -33,9 +33,10 @@ template <typename Real> struct alignas(2 * sizeof(Real) > 8 ? 16 : 8) EdgeKey {
 // Per-window tree arena.  Byte offsets from the window's arena base.
 // Lifetimes (arrays sharing storage are never live together):
 //   fixed   keys[n] (sort pairs; sorted weights feed condensation)
-//           edges[2n] int16: edge_u | edge_v, later the prune queue
-//           (2n entries), later label_map | resolved
-//   region  uf, comp (linkage) -> relabel, queue (condense) -> stop, mark (labels)
+//           edges[2n] int16: edge_u | edge_v, later the per-node prune
+//           clusters (2n entries), later label_map | resolved
+//   region  uf|comp (2n entries: linkage) -> condensation queue (node, cluster)
+//           -> parents of all nodes -> stop, mark (labels)
 //           left, right, sizes (linkage, condense) -> first_child,
 //           next_sibling, blocked (selection)
 //           parent, birth, stability, selected, point_parent
@@ -57,8 +58,8 @@ HDB_HD TreeLayout tree_layout(int n, int real_size) {
   L.edges = o;
   o = align16(o + 4u * un);
   L.region = o;
-  L.uf = o; o = align16(o + 2u * un);
-  L.comp = o; o = align16(o + 2u * un);
+  L.uf = o; o = align16(o + 4u * un);
+  L.comp = L.uf + 2u * un;
   L.left = o; o = align16(o + 2u * un);
   L.right = o; o = align16(o + 2u * un);
   L.sizes = o; o = align16(o + 2u * un);
@@ -68,6 +69,10 @@ HDB_HD TreeLayout tree_layout(int n, int real_size) {
   L.selected = o; o = align16(o + un);
   L.pparent = o; o = align16(o + 2u * un);
   L.stack = o; o = align16(o + 3u * 4u * kSortStack);
+  // The parallel introsort's task lists reuse birth..end (free until
+  // condensation); make sure that span is long enough for tiny windows.
+  const uint32_t lists = 2u * (8u * (un / 4u + 8u) + 2u * (un / 2u + 8u));
+  if (o < L.birth + lists) o = align16(L.birth + lists);
   L.total = o;
   return L;
 }

@@ -78,13 +78,30 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
   if (__any(bad) && (tid & 63) == 0) atomicOr(status, 1);
 }
 
-// Inserts v into the ascending register list lst, dropping the largest.
+// Upper bound of b*b (see square_bound in hdb_prim.h).
+__device__ __forceinline__ float sq_bound(float b) {
+  const float p = b * b;
+  return isfinite(p) ? __uint_as_float(__float_as_uint(p) + 1u) : p;
+}
+__device__ __forceinline__ double sq_bound(double b) {
+  const double p = b * b;
+  return isfinite(p) ? __longlong_as_double(__double_as_longlong(p) + 1ll) : p;
+}
+
+// Offers the squared distance tot to the ascending register list lst of the
+// KMAX smallest distances.  tot >= lim (an upper bound of lst[KMAX-1]^2)
+// means sqrt(tot) >= lst[KMAX-1]: the list cannot change and the correctly
+// rounded square root is skipped.
 template <typename Real, int KMAX>
-__device__ __forceinline__ void topk_insert(Real* lst, Real v) {
-  if (v < lst[KMAX - 1]) {
+__device__ __forceinline__ void topk_offer(Real* lst, Real& lim, Real tot) {
+  if (tot < lim) {
+    const Real v = sqrt(tot);
+    if (v < lst[KMAX - 1]) {
 #pragma unroll
-    for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
-    lst[0] = fmin(lst[0], v);
+      for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
+      lst[0] = fmin(lst[0], v);
+      lim = sq_bound(lst[KMAX - 1]);
+    }
   }
 }
 
@@ -95,7 +112,8 @@ __device__ __forceinline__ void topk_insert(Real* lst, Real v) {
 // 64-point tiles of the window through LDS.  DK > 0: dimensions padded to DK
 // (zero padding adds +0 exactly); DK == 0: any d, 32-dimension chunks.
 template <typename Real, int KMAX, int DK>
-__global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, int d, int ms, Real* core) {
+__global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, int d, int ms, Real* core,
+                                                        int32_t* status) {
   constexpr int TJ = DK > 0 ? 64 : 32;
   constexpr int KC = DK > 0 ? DK : 32;
   __shared__ Real tile[TJ * KC];
@@ -106,6 +124,8 @@ __global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, in
   Real lst[KMAX];
 #pragma unroll
   for (int q = 0; q < KMAX; ++q) lst[q] = Real(INFINITY);
+  Real lim = Real(INFINITY);
+  int bad = 0;
   Real xi[KC];
   if (DK > 0) {
 #pragma unroll
@@ -127,7 +147,8 @@ __global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, in
           const Real dl = xi[k] - tile[jj * KC + k];
           t = t + dl * dl;
         }
-        topk_insert<Real, KMAX>(lst, sqrt(t));
+        bad |= !isfinite(t);
+        topk_offer<Real, KMAX>(lst, lim, t);
       }
     } else {
       Real tot[TJ];
@@ -153,7 +174,10 @@ __global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, in
           tot[jj] = t;
         }
       }
-      for (int jj = 0; jj < jn; ++jj) topk_insert<Real, KMAX>(lst, sqrt(tot[jj]));
+      for (int jj = 0; jj < jn; ++jj) {
+        bad |= !isfinite(tot[jj]);
+        topk_offer<Real, KMAX>(lst, lim, tot[jj]);
+      }
     }
   }
   if (i < n) {
@@ -163,6 +187,10 @@ __global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, in
       if (q == ms - 1) v = lst[q];
     core[(size_t)b * n + i] = v;
   }
+  // Every ordered pair passes through here, so this is the overflow check of
+  // the whole distance matrix (Prim may skip pairs that cannot matter).
+  bad = bad && i < n;
+  if (__any(bad) && (tid & 63) == 0) atomicOr(status, 1);
 }
 
 template <typename Real> struct BitsOf;
@@ -212,6 +240,7 @@ struct TreeArgs {
   long long mcs;
   double epsilon;
   int sort_depth;
+  int serial_sort;       // one-lane introsort instead of the workgroup-parallel one
   int32_t* labels;
   unsigned char* arena;  // global arena (when not in LDS)
   size_t arena_stride;
@@ -223,9 +252,11 @@ struct TreeArgs {
 // Tree stages of one window per workgroup.  Lane 0 runs the sequential
 // stages; the workgroup stages inputs and writes labels and exports.
 template <typename Real, bool LDSA>
-__global__ __launch_bounds__(64) void tree_kernel(TreeArgs a) {
+__global__ __launch_bounds__(256) void tree_kernel(TreeArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int sh_next;
+  __shared__ int sh_counters[4];
+  __shared__ long long sh_clk[6];
   const int b = blockIdx.x, tid = threadIdx.x, T = blockDim.x, n = a.n;
   int32_t* lab = a.labels + (size_t)b * n;
   if (a.wflag[b]) {
@@ -246,18 +277,39 @@ __global__ __launch_bounds__(64) void tree_kernel(TreeArgs a) {
     t.edge_v[i] = (int16_t)ev[i];
   }
   __syncthreads();
-  if (tid == 0) {
-    introsort(t.keys, n - 1, a.sort_depth, t.stack);
-    linkage(t, n);
-    sh_next = condense(t, n, a.mcs);
+  // Phase timestamps (constant 100 MHz clock) for the diagnostic export.
+  long long clk[6];
+  clk[0] = wall_clock64();
+  if (a.serial_sort) {
+    if (tid == 0) introsort(t.keys, n - 1, a.sort_depth, t.stack);
+    __syncthreads();
+  } else {
+    // The lists live in birth..stack, which is free until condensation and
+    // at least sort_lists_bytes(n) long (tree_layout guarantees it).
+    SortLists lists;
+    lists.cap = sort_task_cap(n);
+    lists.cap_small = sort_small_cap(n);
+    lists.cur = reinterpret_cast<int16_t*>(t.birth);
+    lists.next = lists.cur + 3 * lists.cap;
+    lists.heap = lists.next + 3 * lists.cap;
+    lists.small = lists.heap + 2 * lists.cap;
+    parallel_introsort(t.keys, n - 1, a.sort_depth, t.uf, t.comp, lists, sh_counters);
+  }
+  {
+    // parent / point_parent / birth are free until condensation.
+    int16_t* su = t.parent;
+    int16_t* sv = t.pparent;
+    int16_t* rsize = reinterpret_cast<int16_t*>(t.birth);
+    clk[1] = wall_clock64();
+    linkage_gather(t, n, su, sv, rsize);
+    __syncthreads();
+    if (tid == 0) {
+      clk[2] = wall_clock64();
+      linkage(t, n, su, sv, rsize);
+      clk[3] = wall_clock64();
+    }
   }
   __syncthreads();
-  const int nl = sh_next;
-  if (nl < 0) {
-    for (int i = tid; i < n; i += T) lab[i] = -1;
-    if (tid == 0) atomicOr(a.status, 1);
-    return;
-  }
   int32_t* xi = a.export_ints ? a.export_ints + (size_t)b * 10 * n : nullptr;
   Real* xr = a.export_reals ? static_cast<Real*>(a.export_reals) + (size_t)b * 4 * n : nullptr;
   if (xi) {
@@ -271,11 +323,42 @@ __global__ __launch_bounds__(64) void tree_kernel(TreeArgs a) {
       xi[5 * n + i] = e ? t.sizes[i] : -1;
       xr[0 * n + i] = e ? ew[i] : Real(0);
       xr[1 * n + i] = e ? t.keys[i].w : Real(0);
-      xr[2 * n + i] = i < nl - n ? t.birth[i] : Real(0);
     }
+    __syncthreads();
+  }
+  // Condensation scratch reuses arrays dead after linkage: uf|comp (2n
+  // entries) hold the (node, cluster) queue, then the parents of all 2n-1
+  // nodes; the 2n edge endpoints hold the per-node prune clusters.
+  int16_t* prune_cluster = t.edge_u;
+  condense_prepare(t, n, prune_cluster);
+  __syncthreads();
+  if (tid == 0) {
+    sh_next = condense(t, n, a.mcs, t.uf, t.comp, prune_cluster);
+    clk[4] = wall_clock64();
   }
   __syncthreads();
-  if (tid == 0) select_clusters(t, n, nl, (Real)a.epsilon);
+  const int nl = sh_next;
+  if (nl < 0) {
+    for (int i = tid; i < n; i += T) lab[i] = -1;
+    if (tid == 0) atomicOr(a.status, 1);
+    return;
+  }
+  for (int i = tid; i < n - 1; i += T) {
+    t.uf[t.left[i]] = (int16_t)(n + i);
+    t.uf[t.right[i]] = (int16_t)(n + i);
+  }
+  if (tid == 0) t.uf[2 * n - 2] = (int16_t)(2 * n - 2);  // the root is its own parent
+  __syncthreads();
+  condense_points(t, n, t.uf, prune_cluster);
+  if (xr) {
+    for (int i = tid; i < n; i += T) xr[2 * n + i] = i < nl - n ? t.birth[i] : Real(0);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    select_clusters(t, n, nl, (Real)a.epsilon);
+    clk[5] = wall_clock64();
+    for (int q = 0; q < 6; ++q) sh_clk[q] = clk[q];
+  }
   __syncthreads();
   if (xi) {
     for (int i = tid; i < n; i += T) {
@@ -283,7 +366,7 @@ __global__ __launch_bounds__(64) void tree_kernel(TreeArgs a) {
       xi[6 * n + i] = c ? t.parent[i] : -1;
       xi[7 * n + i] = c ? t.selected[i] : 0;
       xi[8 * n + i] = t.pparent[i];
-      xi[9 * n + i] = i == 0 ? nl : 0;
+      xi[9 * n + i] = i == 0 ? nl : (i < 6 ? (int32_t)(sh_clk[i] - sh_clk[i - 1]) : 0);
       xr[3 * n + i] = c ? t.stab[i] : Real(0);
     }
   }

@@ -67,14 +67,81 @@ template <> struct Cand<double> {
   __device__ __forceinline__ double weight() const { return __longlong_as_double((long long)w); }
 };
 
+// DPP lane moves within 16-lane rows (quad_perm xor 1, xor 2; row_ror 4, 8).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRor4 = 0x124, kDppRor8 = 0x128;
+
+__device__ __forceinline__ uint32_t read_lane(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+template <int CTRL>
+__device__ __forceinline__ Cand<float> dpp_cand(const Cand<float>& c) {
+  Cand<float> o;
+  o.k = ((uint64_t)dpp_move<CTRL>((uint32_t)(c.k >> 32)) << 32) | dpp_move<CTRL>((uint32_t)c.k);
+  return o;
+}
+template <int CTRL>
+__device__ __forceinline__ Cand<double> dpp_cand(const Cand<double>& c) {
+  Cand<double> o;
+  o.w = ((uint64_t)dpp_move<CTRL>((uint32_t)(c.w >> 32)) << 32) | dpp_move<CTRL>((uint32_t)c.w);
+  o.j = dpp_move<CTRL>(c.j);
+  return o;
+}
+__device__ __forceinline__ Cand<float> lane_cand(const Cand<float>& c, int lane) {
+  Cand<float> o;
+  o.k = ((uint64_t)read_lane((uint32_t)(c.k >> 32), lane) << 32) | read_lane((uint32_t)c.k, lane);
+  return o;
+}
+__device__ __forceinline__ Cand<double> lane_cand(const Cand<double>& c, int lane) {
+  Cand<double> o;
+  o.w = ((uint64_t)read_lane((uint32_t)(c.w >> 32), lane) << 32) | read_lane((uint32_t)c.w, lane);
+  o.j = read_lane(c.j, lane);
+  return o;
+}
+
+// Minimum within each 16-lane row, left in every lane of the row.
+template <typename Real>
+__device__ __forceinline__ Cand<Real> row_min(Cand<Real> c) {
+  Cand<Real> o = dpp_cand<kDppXor1>(c);
+  if (o.less(c)) c = o;
+  o = dpp_cand<kDppXor2>(c);
+  if (o.less(c)) c = o;
+  o = dpp_cand<kDppRor4>(c);
+  if (o.less(c)) c = o;
+  o = dpp_cand<kDppRor8>(c);
+  if (o.less(c)) c = o;
+  return c;
+}
+
+// Wave-wide minimum, returned wave-uniform (scalar registers).  All 64 lanes
+// must be active.
 template <typename Real>
 __device__ __forceinline__ Cand<Real> wave_min(Cand<Real> c) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-    Cand<Real> o = c.xor_shuffle(m);
-    if (o.less(c)) c = o;
-  }
-  return c;
+  c = row_min(c);
+  Cand<Real> m = lane_cand(c, 0);
+  Cand<Real> o = lane_cand(c, 16);
+  if (o.less(m)) m = o;
+  o = lane_cand(c, 32);
+  if (o.less(m)) m = o;
+  o = lane_cand(c, 48);
+  if (o.less(m)) m = o;
+  return m;
+}
+
+// An upper bound of b*b (exact real square): the product rounded, then one
+// ulp up.  For tot >= square_bound(b), sqrt(tot) >= b exactly, hence the
+// correctly rounded sqrt(tot) >= b.
+__device__ __forceinline__ float square_bound(float b) {
+  const float p = b * b;
+  return isfinite(p) ? __uint_as_float(__float_as_uint(p) + 1u) : p;
+}
+__device__ __forceinline__ double square_bound(double b) {
+  const double p = b * b;
+  return isfinite(p) ? __longlong_as_double(__double_as_longlong(p) + 1ll) : p;
 }
 
 struct PrimArgs {
@@ -125,13 +192,14 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     for (int i = tid; i < n; i += T) cs[i] = cg[i];
 
   Real xr[VPT][DKS];
-  Real best[VPT], cr[VPT];
+  Real best[VPT], cr[VPT], lim[VPT];
   int src[VPT];
   uint32_t vis = 0;
 #pragma unroll
   for (int s = 0; s < VPT; ++s) {
     const int j = tid + s * T;
     best[s] = Real(INFINITY);
+    lim[s] = Real(INFINITY);
     src[s] = 1;
     cr[s] = 0;
 #pragma unroll
@@ -167,9 +235,12 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     C c = C::make(rmax, 0);
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
-      if (!((vis >> s) & 1u)) {
+      // Only a weight strictly below best can change this vertex.  With core
+      // distances, max(core_cur, core_j) >= best rules it out before any
+      // distance is formed (w = max(core_cur, core_j, d) >= best).
+      const bool live = !((vis >> s) & 1u) && (unit || fmax(ccur, cr[s]) < best[s]);
+      if (__ballot(live)) {
         const int j = tid + s * T;
-        Real dd;
         if (MODE == 0) {
           Real tot = 0;
 #pragma unroll
@@ -177,29 +248,40 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
             const Real dl = xr[s][k] - xc[k];
             tot = tot + dl * dl;
           }
-          dd = sqrt(tot);
+          bad |= live && !isfinite(tot);
+          // tot >= lim[s] (an upper bound of best^2) implies sqrt(tot) >= best:
+          // no update, so the correctly rounded square root is skipped.
+          const bool near = live && tot < lim[s];
+          if (__ballot(near)) {
+            const Real dd = sqrt(tot);
+            const Real w = unit ? dd : fmax(fmax(ccur, cr[s]), dd);
+            if (near && w < best[s]) {
+              best[s] = w;
+              src[s] = cur;
+              lim[s] = square_bound(w);
+            }
+          }
         } else {
-          dd = dg[(size_t)cur * n + j];
+          const Real dd = live ? dg[(size_t)cur * n + j] : Real(0);
+          bad |= live && !isfinite(dd);
+          const Real w = unit ? dd : fmax(fmax(ccur, cr[s]), dd);
+          if (live && w < best[s]) {
+            best[s] = w;
+            src[s] = cur;
+          }
         }
-        bad |= !isfinite(dd);
-        Real w = dd;
-        if (!unit) w = fmax(fmax(ccur, cr[s]), dd);
-        if (w < best[s]) {
-          best[s] = w;
-          src[s] = cur;
-        }
-        const C o = C::make(best[s], j);
+      }
+      if (!((vis >> s) & 1u)) {
+        const C o = C::make(best[s], tid + s * T);
         if (o.less(c)) c = o;
       }
     }
     c = wave_min(c);
     if (lane == 0) red[par][wave] = c;
     __syncthreads();
-    C g = red[par][0];
-    for (int w = 1; w < W; ++w) {
-      const C o = red[par][w];
-      if (o.less(g)) g = o;
-    }
+    // Cross-wave minimum: lane w holds wave w's key, one more row reduction.
+    C g = lane < W ? red[par][lane] : C::make(rmax, 0x7fffffff);
+    g = wave_min(g);
     par ^= 1;
     const int nxt = g.index();
 #pragma unroll

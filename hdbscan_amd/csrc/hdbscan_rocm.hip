@@ -25,6 +25,8 @@ using namespace hdb;
 
 namespace {
 
+constexpr int kTreeThreads = 256;
+
 thread_local std::string g_error;
 
 // Optional per-stage HIP event timers of the calling thread (hdb_profile).
@@ -56,16 +58,43 @@ int check_launch(const char* what) {
   return 0;
 }
 
-// Raises a kernel's dynamic LDS limit once per kernel.
+// LDS bytes one workgroup may allocate on the current device (160 KiB on
+// gfx950); falls back to that figure when no device is visible (sizing only).
+uint32_t device_lds_limit() {
+  static std::mutex mu;
+  static int cached_device = -1;
+  static uint32_t cached = kLdsLimit;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return kLdsLimit;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev != cached_device) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0) {
+      cached = (uint32_t)v < kLdsLimit ? (uint32_t)v : kLdsLimit;
+    } else {
+      (void)hipGetLastError();
+      cached = 64u * 1024u;
+    }
+    cached_device = dev;
+  }
+  return cached;
+}
+
+// Raises a kernel's dynamic LDS limit to the bytes a launch needs.  A
+// rejected request is cleared (the launch itself then reports any error).
 int allow_lds(const void* fn, uint32_t bytes) {
   static std::mutex mu;
-  static std::unordered_set<const void*> done;
+  static std::unordered_set<uint64_t> done;
   if (bytes <= 64u * 1024u) return 0;
+  const uint64_t key = ((uint64_t)(uintptr_t)fn << 20) ^ bytes;
   std::lock_guard<std::mutex> lock(mu);
-  if (done.count(fn)) return 0;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
-  if (e != hipSuccess) return fail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  done.insert(fn);
+  if (done.count(key)) return 0;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+    (void)hipGetLastError();
+  done.insert(key);
   return 0;
 }
 
@@ -82,6 +111,7 @@ struct Plan {
   uint32_t tree_lds_bytes = 0;
   size_t arena_stride = 0;
   bool need_matrix = false;
+  int flags = 0;
   size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_matrix = 0, total = 0;
 };
 
@@ -97,9 +127,12 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   if (dtype != HDB_F32 && dtype != HDB_F64) return fail("dtype must be HDB_F32 or HDB_F64");
   if (batch < 1 || d < 1 || n < 2) return fail("X requires nonempty batches/features and at least 2 samples");
   if (n > HDB_MAX_SAMPLES) return fail("at most 4096 samples per window are supported");
-  if (ms < 1 || ms > n) return fail("min_samples must be in [1, n]");
+  // hdb_cluster_matrix sizes with min_samples 3 to reserve core storage.
+  if (ms < 1 || (ms > n && !(flags & HDB_FLAG_MATRIX_INPUT))) return fail("min_samples must be in [1, n]");
   if (d > (1 << 24) || batch > (1 << 24)) return fail("dimension or batch too large");
   Plan P;
+  const uint32_t lds_limit = device_lds_limit();
+  P.flags = flags;
   P.rs = dtype == HDB_F32 ? 4 : 8;
   const int rs = P.rs;
   P.core = ms > 2;
@@ -107,26 +140,35 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.kmax = ms <= 8 ? 8 : ms <= 16 ? 16 : ms <= 32 ? 32 : 64;
   P.core_dk = pad_dims(d);
   // Points path: owned coordinates must fit 64 VGPRs per thread.
-  int want_vpt = (flags >> 8) & 15;
-  int vpt = want_vpt ? want_vpt : (n <= 1024 ? 1 : n <= 2048 ? 2 : 4);
-  if (vpt != 1 && vpt != 2 && vpt != 4) return fail("vertices per thread must be 1, 2 or 4");
+  // Vertices per Prim thread: about 512 threads per window measured best at
+  // N=1024 (fewer waves to synchronise per step, two slots of ILP); larger
+  // windows grow the block up to 1024 threads when registers would not hold
+  // the coordinates of more slots.
+  const int dk = pad_dims(d);
+  P.dk = dk;
+  const int want_vpt = (flags >> 8) & 15;
+  if (want_vpt != 0 && want_vpt != 1 && want_vpt != 2 && want_vpt != 4)
+    return fail("vertices per thread must be 1, 2 or 4");
+  int vpt = want_vpt;
+  if (!vpt) {
+    vpt = n <= 512 ? 1 : n <= 1024 ? 2 : 4;
+    while (vpt > 1 && (int64_t)vpt * dk * rs > 64 * 4 && (int64_t)(vpt / 2) * 1024 >= n) vpt /= 2;
+  }
   while ((int64_t)vpt * 1024 < n) vpt *= 2;
   if (vpt > 4) return fail("window too large for the vertex layout");
   P.vpt = vpt;
   P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
-  const int dk = pad_dims(d);
-  P.dk = dk;
   P.matrix = (flags & HDB_FLAG_MATRIX) || dk == 0 || (int64_t)vpt * dk * rs > 64 * 4;
   const uint32_t core_bytes = P.core ? (uint32_t)(n * rs) : 0u;
   if (!P.matrix) {
     const uint64_t xs = (uint64_t)n * dk * rs;
-    P.xlds = !(flags & HDB_FLAG_NO_POINT_LDS) && xs + core_bytes + 1024 <= kLdsLimit - 2048;
+    P.xlds = !(flags & HDB_FLAG_NO_POINT_LDS) && xs + core_bytes + 3072 <= lds_limit;
     P.prim_lds = (uint32_t)((P.xlds ? xs : 0) + core_bytes);
   } else {
     P.prim_lds = core_bytes;
   }
   const TreeLayout L = tree_layout((int)n, rs);
-  P.tree_lds = !(flags & HDB_FLAG_GLOBAL_ARENA) && L.total + 1024 <= kLdsLimit;
+  P.tree_lds = !(flags & HDB_FLAG_GLOBAL_ARENA) && L.total + 1024 <= lds_limit;
   P.tree_lds_bytes = P.tree_lds ? L.total : 0u;
   P.arena_stride = P.tree_lds ? 0 : align256(L.total);
   P.need_matrix = !(flags & HDB_FLAG_MATRIX_INPUT) && (P.matrix || (P.core && !P.core_topk));
@@ -152,14 +194,15 @@ int launch_distances(hipStream_t s, const Real* x, int64_t batch, int64_t n, int
 }
 
 template <typename Real, int KMAX>
-int launch_topk_k(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_t d, int ms, Real* core, int dk) {
+int launch_topk_k(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_t d, int ms, Real* core, int dk,
+                  int32_t* status) {
   const dim3 grid((unsigned)((n + 255) / 256), (unsigned)batch);
   switch (dk) {
-    case 8: core_topk_kernel<Real, KMAX, 8><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
-    case 16: core_topk_kernel<Real, KMAX, 16><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
-    case 32: core_topk_kernel<Real, KMAX, 32><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
-    case 64: core_topk_kernel<Real, KMAX, 64><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
-    default: core_topk_kernel<Real, KMAX, 0><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core); break;
+    case 8: core_topk_kernel<Real, KMAX, 8><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
+    case 16: core_topk_kernel<Real, KMAX, 16><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
+    case 32: core_topk_kernel<Real, KMAX, 32><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
+    case 64: core_topk_kernel<Real, KMAX, 64><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
+    default: core_topk_kernel<Real, KMAX, 0><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
   }
   return check_launch("core_topk_kernel");
 }
@@ -169,10 +212,10 @@ int launch_core(hipStream_t s, const Plan& P, const Real* x, int64_t batch, int6
                 Real* core, Real* matrix, int32_t* status, bool matrix_ready) {
   if (P.core_topk) {
     switch (P.kmax) {
-      case 8: return launch_topk_k<Real, 8>(s, x, batch, n, d, ms, core, P.core_dk);
-      case 16: return launch_topk_k<Real, 16>(s, x, batch, n, d, ms, core, P.core_dk);
-      case 32: return launch_topk_k<Real, 32>(s, x, batch, n, d, ms, core, P.core_dk);
-      default: return launch_topk_k<Real, 64>(s, x, batch, n, d, ms, core, P.core_dk);
+      case 8: return launch_topk_k<Real, 8>(s, x, batch, n, d, ms, core, P.core_dk, status);
+      case 16: return launch_topk_k<Real, 16>(s, x, batch, n, d, ms, core, P.core_dk, status);
+      case 32: return launch_topk_k<Real, 32>(s, x, batch, n, d, ms, core, P.core_dk, status);
+      default: return launch_topk_k<Real, 64>(s, x, batch, n, d, ms, core, P.core_dk, status);
     }
   }
   if (!matrix_ready) {
@@ -227,9 +270,9 @@ int launch_tree(hipStream_t s, const Plan& P, int64_t batch, const TreeArgs& a) 
   if (P.tree_lds) {
     auto fn = tree_kernel<Real, true>;
     if (int r = allow_lds((const void*)fn, P.tree_lds_bytes)) return r;
-    fn<<<(unsigned)batch, 64, P.tree_lds_bytes, s>>>(a);
+    fn<<<(unsigned)batch, kTreeThreads, P.tree_lds_bytes, s>>>(a);
   } else {
-    tree_kernel<Real, false><<<(unsigned)batch, 64, 0, s>>>(a);
+    tree_kernel<Real, false><<<(unsigned)batch, kTreeThreads, 0, s>>>(a);
   }
   return check_launch("tree_kernel");
 }
@@ -285,6 +328,7 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   ta.mcs = mcs;
   ta.epsilon = eps;
   ta.sort_depth = sort_depth;
+  ta.serial_sort = (P.flags & HDB_FLAG_SERIAL_SORT) ? 1 : 0;
   ta.labels = labels;
   ta.arena = ws + P.off_arena;
   ta.arena_stride = P.arena_stride;

@@ -47,6 +47,7 @@ extern "C" {
 #define HDB_FLAG_MATRIX 2       /* materialise distances, Prim reads rows */
 #define HDB_FLAG_NO_POINT_LDS 4 /* current vertex read from device memory */
 #define HDB_FLAG_MATRIX_INPUT 8 /* sizing for hdb_cluster_matrix (distances given) */
+#define HDB_FLAG_SERIAL_SORT 16 /* one-lane introsort (the parallel one is default) */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic. */
 
 /* Library version (major*10000 + minor*100 + patch). */

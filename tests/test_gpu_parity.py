@@ -84,13 +84,14 @@ def test_introsort_order_matches_numpy(index):
     np.testing.assert_array_equal(ex[0]["order"], expected)
 
 
-@pytest.mark.parametrize("size", [33, 129, 1024])
+@pytest.mark.parametrize("size", [33, 129, 1024, 4096])
 @pytest.mark.parametrize("depth", [-1, 0, 1, 3])
-def test_forced_sort_depth_matches_oracle(size, depth):
+@pytest.mark.parametrize("serial", [0, 16])
+def test_forced_sort_depth_matches_oracle(size, depth, serial):
     rng = np.random.default_rng(size)
     w = rng.integers(1, 4, size - 1).astype(np.float32)
     dist = path_graph(w, np.float32)
-    _, status, ex = G.cluster_matrix(dist, None, 2, 0.0, "float32", depth=depth)
+    _, status, ex = G.cluster_matrix(dist, None, 2, 0.0, "float32", depth=depth, flags=serial)
     assert status == 0
     np.testing.assert_array_equal(ex[0]["order"], oracle.argsort(w, depth=depth))
 
@@ -191,7 +192,7 @@ def test_workload_windows(index):
     assert same_partition(got, sk_labels)
 
 
-@pytest.mark.parametrize("flags", [0, 1, 2, 4, 2 | 1, 1 << 8, 2 << 8, 4 << 8])
+@pytest.mark.parametrize("flags", [0, 1, 2, 4, 16, 2 | 1, 1 << 8, 2 << 8, 4 << 8])
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_path_variants_agree(flags, precision):
     rng = np.random.default_rng(11)
