@@ -11,8 +11,9 @@ LIBRARY = os.path.join(HERE, "_lib", "libhdbscan_rocm.so")
 F32, F64 = 0, 1
 STATUS_OVERFLOW, STATUS_NONFINITE = 1, 2
 FLAG_GLOBAL_ARENA, FLAG_MATRIX, FLAG_NO_POINT_LDS, FLAG_MATRIX_INPUT, FLAG_SERIAL_SORT = 1, 2, 4, 8, 16
+FLAG_PREFILTER = 32
 EXPORTED = ("hdb_version", "hdb_last_error", "hdb_profile", "hdb_stage_count", "hdb_stage_name",
-            "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_distances",
+            "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_cluster_export", "hdb_distances",
             "hdb_core_distances", "hdb_cluster_matrix")
 
 _lib = None
@@ -48,6 +49,9 @@ def lib():
         handle.hdb_cluster.restype = _I
         handle.hdb_cluster.argtypes = [_P, _P, _I64, _I64, _I64, _I, _I64, _I32, ctypes.c_double,
                                        _P, _P, ctypes.c_size_t, _P, _I]
+        handle.hdb_cluster_export.restype = _I
+        handle.hdb_cluster_export.argtypes = [_P, _P, _I64, _I64, _I64, _I, _I64, _I32, ctypes.c_double, _I,
+                                              _P, _P, ctypes.c_size_t, _P, _I, _P, _P]
         handle.hdb_distances.restype = _I
         handle.hdb_distances.argtypes = [_P, _P, _I64, _I64, _I64, _I, _P, _P]
         handle.hdb_core_distances.restype = _I

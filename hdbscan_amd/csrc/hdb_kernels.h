@@ -244,6 +244,8 @@ struct TreeArgs {
   int32_t* labels;
   unsigned char* arena;  // global arena (when not in LDS)
   size_t arena_stride;
+  unsigned char* slots;  // per-window cluster records of the condensation walks
+  size_t slot_stride;
   int32_t* status;
   int32_t* export_ints;  // [B][10][n] or null
   void* export_reals;    // [B][4][n] or null
@@ -326,30 +328,23 @@ __global__ __launch_bounds__(256) void tree_kernel(TreeArgs a) {
     }
     __syncthreads();
   }
-  // Condensation scratch reuses arrays dead after linkage: uf|comp (2n
-  // entries) hold the (node, cluster) queue, then the parents of all 2n-1
-  // nodes; the 2n edge endpoints hold the per-node prune clusters.
-  int16_t* prune_cluster = t.edge_u;
-  condense_prepare(t, n, prune_cluster);
+  // Condensation scratch (arrays dead after linkage): edge_u holds the slot
+  // of each visited node, parent the pruned point counts (until the final
+  // cluster arrays are written), uf|comp the split keys (n int32) and then
+  // the parents of all 2n-1 nodes.
+  int16_t* node_slot = t.edge_u;
+  condense_prepare(t, n, a.mcs, node_slot, t.parent);
   __syncthreads();
-  if (tid == 0) {
-    sh_next = condense(t, n, a.mcs, t.uf, t.comp, prune_cluster);
-    clk[4] = wall_clock64();
-  }
-  __syncthreads();
-  const int nl = sh_next;
-  if (nl < 0) {
-    for (int i = tid; i < n; i += T) lab[i] = -1;
-    if (tid == 0) atomicOr(a.status, 1);
-    return;
-  }
+  ClusterSlot<Real>* slots = reinterpret_cast<ClusterSlot<Real>*>(a.slots + (size_t)b * a.slot_stride);
+  const int nl = condense_walks(t, n, slots, node_slot, t.parent, reinterpret_cast<int32_t*>(t.uf), sh_counters);
+  if (tid == 0) clk[4] = wall_clock64();
   for (int i = tid; i < n - 1; i += T) {
     t.uf[t.left[i]] = (int16_t)(n + i);
     t.uf[t.right[i]] = (int16_t)(n + i);
   }
   if (tid == 0) t.uf[2 * n - 2] = (int16_t)(2 * n - 2);  // the root is its own parent
   __syncthreads();
-  condense_points(t, n, t.uf, prune_cluster);
+  condense_points(t, n, t.uf, node_slot, slots);
   if (xr) {
     for (int i = tid; i < n; i += T) xr[2 * n + i] = i < nl - n ? t.birth[i] : Real(0);
   }

@@ -132,6 +132,26 @@ __device__ __forceinline__ Cand<Real> wave_min(Cand<Real> c) {
   return m;
 }
 
+// Unit roundoff of Real.
+template <typename Real> struct Unit;
+template <> struct Unit<float> { static constexpr float u = 5.9604644775390625e-08f; };  // 2^-24
+template <> struct Unit<double> { static constexpr double u = 1.1102230246251565e-16; };  // 2^-53
+
+template <typename Real> struct BitsOfReal;
+template <> struct BitsOfReal<float> {
+  __device__ static uint64_t get(float v) { return __float_as_uint(v); }
+  __device__ static float put(uint64_t u) { return __uint_as_float((uint32_t)u); }
+};
+template <> struct BitsOfReal<double> {
+  __device__ static uint64_t get(double v) { return (uint64_t)__double_as_longlong(v); }
+  __device__ static double put(uint64_t u) { return __longlong_as_double((long long)u); }
+};
+// Largest squared norm for which no squared distance of the window can
+// overflow (sum of squared differences <= 2 (|x|^2 + |y|^2) <= 4 * limit).
+template <typename Real> struct GramLimit;
+template <> struct GramLimit<float> { static constexpr float value = 1.2676506002282294e30f; };  // 2^100
+template <> struct GramLimit<double> { static constexpr double value = 8.6e270; };
+
 // An upper bound of b*b (exact real square): the product rounded, then one
 // ulp up.  For tot >= square_bound(b), sqrt(tot) >= b exactly, hence the
 // correctly rounded sqrt(tot) >= b.
@@ -150,6 +170,7 @@ struct PrimArgs {
   const void* core;   // [B][n] or null when unit_core
   int n, d;
   int unit_core;
+  int prefilter;      // certified Gram prefilter before exact distances
   int32_t* edge_u;    // [B][n]
   int32_t* edge_v;    // [B][n]
   void* edge_w;       // [B][n] Real
@@ -170,6 +191,8 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   const bool unit = a.unit_core != 0;
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ C red[2][16];
+  __shared__ unsigned long long kmin[3];
+  __shared__ unsigned long long sh_nmax;
   __shared__ int sh_err;
   constexpr int DKS = DK > 0 ? DK : 1;
   Real* xs = reinterpret_cast<Real*>(lds);
@@ -177,7 +200,11 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   const Real* xg = static_cast<const Real*>(a.x) + (size_t)b * n * (MODE == 0 ? d : 0);
   const Real* dg = static_cast<const Real*>(a.dist) + (size_t)b * n * (MODE == 1 ? n : 0);
   const Real* cg = static_cast<const Real*>(a.core) + (size_t)b * n;
-  if (tid == 0) sh_err = 0;
+  if (tid == 0) {
+    sh_err = 0;
+    sh_nmax = 0;
+    kmin[0] = kmin[1] = kmin[2] = ~0ull;
+  }
   __syncthreads();
   int err = 0;
   if constexpr (MODE == 0 && XLDS) {
@@ -188,11 +215,12 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       xs[idx] = v;
     }
   }
+  Real* xn = cs + (unit ? 0 : n);  // squared norms of all points (points path)
   if (!unit)
     for (int i = tid; i < n; i += T) cs[i] = cg[i];
 
   Real xr[VPT][DKS];
-  Real best[VPT], cr[VPT], lim[VPT];
+  Real best[VPT], cr[VPT], lim[VPT], nr[VPT];
   int src[VPT];
   uint32_t vis = 0;
 #pragma unroll
@@ -217,10 +245,35 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     } else {
       vis |= 1u << s;
     }
+    Real nn = 0;
+#pragma unroll
+    for (int k = 0; k < DKS; ++k) nn = fma(xr[s][k], xr[s][k], nn);
+    nr[s] = nn;
   }
   if (tid == 0) vis |= 1u;  // vertex 0 starts the tree
   if (err) atomicOr(&sh_err, err);
+  // Gram prefilter: norms of every point (for the current vertex) and the
+  // window-wide largest norm.  With every squared norm <= 2^100 (fp32) no
+  // pair can overflow, so skipping pairs never hides an overflow.
+  Real nmax = 0;
+  if (MODE == 0) {
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      if (j < n) {
+        xn[j] = nr[s];
+        nmax = fmax(nmax, nr[s]);
+      }
+    }
+  }
+  // Non-negative IEEE values order like their bit patterns.
+  if (MODE == 0) atomicMax(&sh_nmax, (unsigned long long)BitsOfReal<Real>::get(nmax));
   __syncthreads();
+  const bool gram = MODE == 0 && a.prefilter && BitsOfReal<Real>::put(sh_nmax) <= GramLimit<Real>::value;
+  // Bound on |a - r| for a = fl(|x_j|^2 + |x_c|^2 - 2 x_j.x_c) (FMA chains of
+  // DK terms) and r the exact real squared distance: 2.03 (DK + 3) u
+  // (n_j + n_c) is enough; E uses 2 (DK + 4) u and the test below doubles it.
+  const Real gram_k = Real(2 * (DKS + 4)) * Unit<Real>::u;
 
   const Real rmax = RealTraits<Real>::max_value();
   int cur = 0, par = 0, bad = 0;
@@ -232,6 +285,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         xc[k] = XLDS ? xs[cur * DK + k] : (k < d ? xg[(size_t)cur * d + k] : Real(0));
     }
     const Real ccur = unit ? Real(0) : cs[cur];
+    const Real ncur = gram ? xn[cur] : Real(0);
     C c = C::make(rmax, 0);
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
@@ -239,7 +293,19 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       // distances, max(core_cur, core_j) >= best rules it out before any
       // distance is formed (w = max(core_cur, core_j, d) >= best).
       const bool live = !((vis >> s) & 1u) && (unit || fmax(ccur, cr[s]) < best[s]);
-      if (__ballot(live)) {
+      bool need = live;
+      if (gram && __ballot(live)) {
+        // Certified skip: a - 2E >= lim (1 + 2^-16) implies the exact
+        // coordinate-order sum of squares is >= lim, hence no update.
+        Real g = 0;
+#pragma unroll
+        for (int k = 0; k < DKS; ++k) g = fma(xr[s][k], xc[k], g);
+        const Real sn = nr[s] + ncur;
+        const Real approx = sn - Real(2) * g;
+        const Real err = gram_k * sn;
+        need = live && !(approx - Real(2) * err >= lim[s] * Real(1.0000152587890625));
+      }
+      if (__ballot(need)) {
         const int j = tid + s * T;
         if (MODE == 0) {
           Real tot = 0;
@@ -277,12 +343,24 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       }
     }
     c = wave_min(c);
-    if (lane == 0) red[par][wave] = c;
-    __syncthreads();
-    // Cross-wave minimum: lane w holds wave w's key, one more row reduction.
-    C g = lane < W ? red[par][lane] : C::make(rmax, 0x7fffffff);
-    g = wave_min(g);
-    par ^= 1;
+    C g;
+    if constexpr (sizeof(Real) == 4) {
+      // One 64-bit LDS atomic min per wave; slot step%3 is reset two steps
+      // ahead (its last reads happened before the previous barrier, its
+      // next atomics come after the next one), so one barrier per step.
+      if (lane == 0) atomicMin(&kmin[par], c.k);
+      __syncthreads();
+      g.k = kmin[par];
+      if (tid == 0) kmin[par == 0 ? 2 : par - 1] = ~0ull;
+      par = par == 2 ? 0 : par + 1;
+    } else {
+      if (lane == 0) red[par & 1][wave] = c;
+      __syncthreads();
+      // Cross-wave minimum: lane w holds wave w's key, one more reduction.
+      g = lane < W ? red[par & 1][lane] : C::make(rmax, 0x7fffffff);
+      g = wave_min(g);
+      par ^= 1;
+    }
     const int nxt = g.index();
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {

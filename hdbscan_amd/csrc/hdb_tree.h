@@ -377,6 +377,7 @@ __device__ void linkage_gather(const TreeView<Real>& t, int n, int16_t* su, int1
   }
 }
 
+// Run by one lane; see linkage_gather for the inputs.
 template <typename Real>
 __device__ void linkage(const TreeView<Real>& t, int n, const int16_t* su, const int16_t* sv, int16_t* rsize) {
   int16_t* uf = t.uf;
@@ -414,99 +415,193 @@ __device__ void linkage(const TreeView<Real>& t, int n, const int16_t* su, const
 }
 
 // Condensation, prepared in parallel.  After linkage the sort index of each
-// key is dead: keys[i] becomes {lambda of node n+i, child sizes packed
-// size_left | size_right << 16}.
+// key is dead; for node n+i:
+//   keys[i].w   = lambda = 1/distance (+inf for distance 0)
+//   keys[i].idx = next | pre_offset << 13 | split << 27, where next is the
+//                 child that keeps the node's cluster (its only child with
+//                 >= mcs points; 0 if none), pre_offset that child's preorder
+//                 offset (1 for the left child, 1 + nodes(left) = 2*size(left)
+//                 for the right) and split marks two such children
+//   pruned[i]   = points in children with < mcs points (pruned at this node)
+//   node_slot[i] = -1 (set to the cluster slot when a walk visits the node)
 template <typename Real>
-__device__ void condense_prepare(const TreeView<Real>& t, int n, int16_t* prune_cluster) {
+__device__ void condense_prepare(const TreeView<Real>& t, int n, long long mcs, int16_t* node_slot,
+                                 int16_t* pruned) {
   for (int i = threadIdx.x; i < n - 1; i += blockDim.x) {
     const int a = t.left[i], c = t.right[i];
     const int sa = a < n ? 1 : t.sizes[a - n];
     const int sc = c < n ? 1 : t.sizes[c - n];
+    const bool la = sa >= mcs, lc = sc >= mcs;
     const Real dist = t.keys[i].w;
+    const int next = la && !lc ? a : (lc && !la ? c : 0);
+    const int off = la && !lc ? 1 : 2 * sa;
     EdgeKey<Real> k;
     k.w = dist > Real(0) ? Real(1) / dist : Real(INFINITY);
-    k.idx = sa | (sc << 16);
+    k.idx = next | (off << 13) | ((la && lc) ? (1 << 27) : 0);
     t.keys[i] = k;
-  }
-  for (int x = threadIdx.x; x < 2 * n; x += blockDim.x) prune_cluster[x] = -1;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    t.parent[i] = -1;
-    t.selected[i] = 0;
-    t.birth[i] = 0;
-    t.stab[i] = 0;
+    pruned[i] = (int16_t)((la ? 0 : sa) + (lc ? 0 : sc));
+    node_slot[i] = -1;
   }
 }
 
-// Breadth-first condensation from root 2n-2 (cluster n), visiting only the
-// nodes that stay in the condensed tree (the root and nodes of >= mcs
-// points).  Children of a split with both sides >= mcs get consecutive new
-// cluster IDs; a smaller child is pruned: each of its `size` points adds
-// lambda - birth to the cluster's stability, one addition at a time in the
-// reference's order (all its points share the same lambda), and the subtree
-// root remembers its cluster for condense_points.  The queue carries
-// (node, cluster) pairs.  Returns next_label, or -1 on a queue overflow
-// (malformed tree).
+// Per-cluster record of the cluster walks (device memory, one per cluster).
+template <typename Real> struct ClusterSlot {
+  Real birth;        // lambda at which the cluster appears (0 for the root)
+  Real stab;         // stability accumulated along the cluster's path
+  int32_t parent;    // parent slot (-1 for the root)
+  int32_t key;       // depth << 13 | preorder of the node that splits it, -1 if none
+  int32_t root;      // first node of the cluster's path
+  int32_t depth;     // depth of that node in the single-linkage tree
+  int32_t pre;       // its preorder index (left subtree first)
+  int32_t label;     // final cluster id
+};
+
+// Condensation by cluster walks (after condense_prepare).  The nodes kept by the reference's
+// breadth-first condensation form, per cluster, one downward path: from the
+// cluster's first node, each node has at most one child of >= mcs points that
+// inherits the cluster, and the path ends at a node whose children are both
+// >= mcs (a split: two new clusters) or both smaller.  The reference adds to
+// a cluster's stability only while visiting that path, in path order, so a
+// lane can accumulate it alone: each smaller child adds lambda - birth once
+// per point (all its points leave at the same lambda), a split adds
+// (lambda - birth) * size for the left then the right child.  Walks run level
+// by level (children clusters start after their parent's walk).  The
+// reference numbers new clusters in breadth-first order of the split nodes,
+// i.e. by (depth, preorder) of those nodes: split number r gives its children
+// n+1+2r (left) and n+2+2r (right).  node_slot[i] records the slot of each
+// visited node n+i.
+// Called by every thread of the workgroup; returns next_label (uniform).
 template <typename Real>
-__device__ int condense(const TreeView<Real>& t, int n, long long mcs, int16_t* qnode, int16_t* qclus,
-                        int16_t* prune_cluster) {
-  const int root = 2 * n - 2;
-  int head = 0, tail = 1, next_label = n + 1;
-  qnode[0] = (int16_t)root;
-  qclus[0] = (int16_t)n;
-  while (head < tail) {
-    const int node = qnode[head], cluster = qclus[head];
-    ++head;
-    const int i = node - n;
-    const EdgeKey<Real> info = t.keys[i];
-    const int a = t.left[i], c = t.right[i];
-    const Real cluster_birth = t.birth[cluster - n];
-    Real stab = t.stab[cluster - n];
-    const Real lambda = info.w;
-    const int size_a = info.idx & 0xffff, size_c = (info.idx >> 16) & 0xffff;
-    const bool large_a = size_a >= mcs, large_c = size_c >= mcs;
-    if (large_a && large_c) {
-      if (tail + 2 > n || next_label + 2 > 2 * n) return -1;
-      for (int k = 0; k < 2; ++k) {
-        const int child = k == 0 ? a : c;
-        const int label = next_label++;
-        t.parent[label - n] = (int16_t)cluster;
-        t.birth[label - n] = lambda;
-        t.selected[label - n] = 1;
-        stab += (lambda - cluster_birth) * Real(k == 0 ? size_a : size_c);
-        qnode[tail] = (int16_t)child;
-        qclus[tail] = (int16_t)label;
-        ++tail;
-      }
-    } else {
-      const Real step = lambda - cluster_birth;
-      for (int k = 0; k < 2; ++k) {
-        const int child = k == 0 ? a : c;
-        if (k == 0 ? large_a : large_c) {
-          if (tail + 1 > n) return -1;
-          qnode[tail] = (int16_t)child;
-          qclus[tail] = (int16_t)cluster;
-          ++tail;
-        } else {
-          prune_cluster[child] = (int16_t)cluster;
-          const int size = k == 0 ? size_a : size_c;
-          for (int q = 0; q < size; ++q) stab += step;
-        }
-      }
-    }
-    t.stab[cluster - n] = stab;
+__device__ int condense_walks(const TreeView<Real>& t, int n, ClusterSlot<Real>* slots, int16_t* node_slot,
+                              const int16_t* pruned, int32_t* split_keys, int* counters) {
+  // counters: [0] level begin, [1] level end, [2] slots allocated, [3] splits
+  const int tid = threadIdx.x, T = blockDim.x;
+  if (tid == 0) {
+    ClusterSlot<Real> r;
+    r.birth = 0;
+    r.stab = 0;
+    r.parent = -1;
+    r.key = -1;
+    r.root = 2 * n - 2;
+    r.depth = 0;
+    r.pre = 0;
+    r.label = n;
+    slots[0] = r;
+    counters[0] = 0;
+    counters[1] = 1;
+    counters[2] = 1;
   }
-  return next_label;
+  __syncthreads();
+  for (;;) {
+    const int begin = counters[0], end = counters[1];
+    if (begin >= end) break;
+    for (int s = begin + tid; s < end; s += T) {
+      int x = slots[s].root, d = slots[s].depth, p = slots[s].pre;
+      const Real b = slots[s].birth;
+      Real stab = 0;
+      int key = -1;
+      for (;;) {
+        if (x < n || x > 2 * n - 2) break;  // only reachable for a malformed tree
+        const int i = x - n;
+        const EdgeKey<Real> info = t.keys[i];
+        const int count = pruned[i];
+        node_slot[i] = (int16_t)s;
+        const Real lambda = info.w;
+        if ((info.idx >> 27) & 1) {
+          const int a = t.left[i], c = t.right[i];
+          const int sa = a < n ? 1 : t.sizes[a - n];
+          const int sc = c < n ? 1 : t.sizes[c - n];
+          stab += (lambda - b) * Real(sa);
+          stab += (lambda - b) * Real(sc);
+          const int s1 = atomicAdd(&counters[2], 2);
+          if (s1 + 1 < n) {
+            ClusterSlot<Real> ca, cc;
+            ca.birth = cc.birth = lambda;
+            ca.stab = cc.stab = 0;
+            ca.parent = cc.parent = s;
+            ca.key = cc.key = -1;
+            ca.depth = cc.depth = d + 1;
+            ca.label = cc.label = -1;
+            ca.root = a;
+            ca.pre = p + 1;
+            cc.root = c;
+            cc.pre = p + 2 * sa;
+            slots[s1] = ca;
+            slots[s1 + 1] = cc;
+          }
+          key = (d << 13) | p;
+          break;
+        }
+        const Real step = lambda - b;
+        for (int q = 0; q < count; ++q) stab += step;
+        const int next = info.idx & 0x1fff;
+        if (!next) break;
+        p += (info.idx >> 13) & 0x3fff;
+        ++d;
+        x = next;
+      }
+      slots[s].stab = stab;
+      slots[s].key = key;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      counters[0] = end;
+      counters[1] = min(counters[2], n);
+    }
+    __syncthreads();
+  }
+  const int nslots = min(counters[2], n);
+  // Rank the splits by (depth, preorder): stage the keys in LDS, count.
+  for (int s = tid; s < nslots; s += T) split_keys[s] = slots[s].key;
+  __syncthreads();
+  for (int s = tid; s < nslots; s += T) {
+    const int k = split_keys[s];
+    if (k < 0) continue;
+    int r = 0;
+    for (int q = 0; q < nslots; ++q) {
+      const int kq = split_keys[q];
+      r += (kq >= 0 && kq < k) ? 1 : 0;
+    }
+    slots[s].label = r;  // rank for now
+  }
+  __syncthreads();
+  int splits = 0;
+  for (int s = tid; s < nslots; s += T) splits += split_keys[s] >= 0 ? 1 : 0;
+  if (tid == 0) counters[3] = 0;
+  __syncthreads();
+  if (splits) atomicAdd(&counters[3], splits);
+  // Labels of children: n + 1 + 2 * rank(parent) + (second child).
+  for (int s = tid; s < nslots; s += T) {
+    if (s == 0) continue;
+    const int par = slots[s].parent;
+    split_keys[s] = n + 1 + 2 * slots[par].label + ((s - 1) & 1);
+  }
+  if (tid == 0) split_keys[0] = n;
+  __syncthreads();
+  // Final cluster arrays, indexed by label - n.
+  for (int s = tid; s < nslots; s += T) {
+    const int lab = split_keys[s];
+    const int c = lab - n;
+    t.birth[c] = slots[s].birth;
+    t.stab[c] = slots[s].stab;
+    t.parent[c] = (int16_t)(s == 0 ? -1 : split_keys[slots[s].parent]);
+    t.selected[c] = (uint8_t)(s == 0 ? 0 : 1);
+    slots[s].label = lab;
+  }
+  __syncthreads();
+  return n + 1 + 2 * counters[3];
 }
 
-// Point parents: each point climbs to the root of the pruned subtree that
-// contains it and takes the cluster recorded there.
+// Point parents: each point climbs to its first ancestor visited by a walk
+// (the node that pruned it) and takes that node's cluster.
 template <typename Real>
 __device__ void condense_points(const TreeView<Real>& t, int n, const int16_t* parent_of,
-                                const int16_t* prune_cluster) {
+                                const int16_t* node_slot, const ClusterSlot<Real>* slots) {
   for (int leaf = threadIdx.x; leaf < n; leaf += blockDim.x) {
-    int x = leaf, guard = 0;
-    while (prune_cluster[x] < 0 && guard++ < 2 * n) x = parent_of[x];
-    t.pparent[leaf] = prune_cluster[x];
+    int x = parent_of[leaf], guard = 0;
+    while (x >= n && node_slot[x - n] < 0 && guard++ < n) x = parent_of[x];
+    const int s = x >= n ? node_slot[x - n] : -1;
+    t.pparent[leaf] = (int16_t)(s >= 0 ? slots[s].label : -1);
   }
 }
 

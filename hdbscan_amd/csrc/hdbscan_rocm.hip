@@ -9,7 +9,8 @@
 //   [tree]  tree_kernel: sort, linkage, condensation, selection, labels
 // Workspace (caller-owned device memory, hdb_workspace_bytes):
 //   edge_u[B][n] i32 | edge_v[B][n] i32 | edge_w[B][n] Real | wflag[B] i32 |
-//   core[B][n] Real | tree arena[B][stride] | distances[B][n][n] Real
+//   core[B][n] Real | tree arena[B][stride] | cluster slots[B][n] |
+//   distances[B][n][n] Real
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -110,9 +111,10 @@ struct Plan {
   bool tree_lds = true;
   uint32_t tree_lds_bytes = 0;
   size_t arena_stride = 0;
+  size_t slot_stride = 0;
   bool need_matrix = false;
   int flags = 0;
-  size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_matrix = 0, total = 0;
+  size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0, off_matrix = 0, total = 0;
 };
 
 int pad_dims(int64_t d) {
@@ -161,9 +163,11 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.matrix = (flags & HDB_FLAG_MATRIX) || dk == 0 || (int64_t)vpt * dk * rs > 64 * 4;
   const uint32_t core_bytes = P.core ? (uint32_t)(n * rs) : 0u;
   if (!P.matrix) {
+    // points (optional) | core distances (when needed) | squared norms
     const uint64_t xs = (uint64_t)n * dk * rs;
-    P.xlds = !(flags & HDB_FLAG_NO_POINT_LDS) && xs + core_bytes + 3072 <= lds_limit;
-    P.prim_lds = (uint32_t)((P.xlds ? xs : 0) + core_bytes);
+    const uint64_t norms = (uint64_t)n * rs;
+    P.xlds = !(flags & HDB_FLAG_NO_POINT_LDS) && xs + core_bytes + norms + 3072 <= lds_limit;
+    P.prim_lds = (uint32_t)((P.xlds ? xs : 0) + core_bytes + norms);
   } else {
     P.prim_lds = core_bytes;
   }
@@ -180,6 +184,8 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.off_flag = o; o = align256(o + (size_t)batch * 4);
   P.off_core = o; o = align256(o + (P.core ? bn * rs : 0));
   P.off_arena = o; o = align256(o + (size_t)batch * P.arena_stride);
+  P.slot_stride = align256((size_t)n * (rs == 4 ? sizeof(ClusterSlot<float>) : sizeof(ClusterSlot<double>)));
+  P.off_slots = o; o = align256(o + (size_t)batch * P.slot_stride);
   P.off_matrix = o; o = align256(o + (P.need_matrix ? bn * (size_t)n * rs : 0));
   P.total = o;
   *p = P;
@@ -312,6 +318,7 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   pa.n = (int)n;
   pa.d = (int)d;
   pa.unit_core = unit ? 1 : 0;
+  pa.prefilter = (P.flags & HDB_FLAG_PREFILTER) ? 1 : 0;
   pa.edge_u = eu;
   pa.edge_v = ev;
   pa.edge_w = ew;
@@ -332,6 +339,8 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   ta.labels = labels;
   ta.arena = ws + P.off_arena;
   ta.arena_stride = P.arena_stride;
+  ta.slots = ws + P.off_slots;
+  ta.slot_stride = P.slot_stride;
   ta.status = status;
   ta.export_ints = exp_i;
   ta.export_reals = exp_r;
@@ -391,6 +400,27 @@ int hdb_cluster(void* stream, const void* x, int64_t batch, int64_t n, int64_t d
                               min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
   return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
                              min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
+}
+
+int hdb_cluster_export(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,
+                       int64_t min_cluster_size, int32_t min_samples, double epsilon, int sort_depth,
+                       int32_t* labels, void* workspace, size_t workspace_bytes, int32_t* status, int flags,
+                       int32_t* export_ints, void* export_reals) {
+  g_error.clear();
+  if (!x || !labels || !status) return fail("null pointer argument");
+  if (min_cluster_size < 2) return fail("min_cluster_size must be >= 2");
+  if ((export_ints == nullptr) != (export_reals == nullptr)) return fail("export buffers must be given together");
+  Plan P;
+  if (make_plan(batch, n, d, dtype, min_samples, flags, &P)) return -1;
+  if (workspace_bytes < P.total || (P.total && !workspace)) return fail("workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
+  const bool unit = min_samples <= 2;
+  if (dtype == HDB_F32)
+    return run_cluster<float>(s, P, static_cast<const float*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+                              min_samples, unit, epsilon, sort_depth, labels, ws, status, export_ints, export_reals);
+  return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+                             min_samples, unit, epsilon, sort_depth, labels, ws, status, export_ints, export_reals);
 }
 
 int hdb_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype, void* out,

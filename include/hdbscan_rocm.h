@@ -48,6 +48,7 @@ extern "C" {
 #define HDB_FLAG_NO_POINT_LDS 4 /* current vertex read from device memory */
 #define HDB_FLAG_MATRIX_INPUT 8 /* sizing for hdb_cluster_matrix (distances given) */
 #define HDB_FLAG_SERIAL_SORT 16 /* one-lane introsort (the parallel one is default) */
+#define HDB_FLAG_PREFILTER 32   /* Prim: certified FMA Gram bound skips exact distances */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic. */
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -81,6 +82,17 @@ int hdb_cluster(void *stream, const void *x, int64_t batch, int64_t n, int64_t d
                 int dtype, int64_t min_cluster_size, int32_t min_samples,
                 double epsilon, int32_t *labels, void *workspace,
                 size_t workspace_bytes, int32_t *status, int flags);
+
+/* hdb_cluster with every intermediate exported (layout of hdb_cluster_matrix
+ * below) and an optional forced introsort depth: the white-box view of the
+ * production path.  Replaces the reference's diagnostic kernels that export
+ * the cluster kernel's workspace (reference tests/test_kernels.py:197-266). */
+int hdb_cluster_export(void *stream, const void *x, int64_t batch, int64_t n,
+                       int64_t d, int dtype, int64_t min_cluster_size,
+                       int32_t min_samples, double epsilon, int sort_depth,
+                       int32_t *labels, void *workspace, size_t workspace_bytes,
+                       int32_t *status, int flags, int32_t *export_ints,
+                       void *export_reals);
 
 /* Dense pairwise distances out[batch][n][n] (diagonal +0), bitwise equal to
  * the reference's triangle-and-mirror kernel.

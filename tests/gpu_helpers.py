@@ -103,6 +103,43 @@ def cluster(x, mcs, ms, eps, precision, flags=0):
     return (out[0] if squeeze else out), int(host(status)[0])
 
 
+def cluster_export(x, mcs, ms, eps, precision, depth=None, flags=0):
+    """hdb_cluster_export: labels, status and per-window intermediates."""
+    t = torch()
+    x = np.asarray(x, real(precision))
+    if x.ndim == 2:
+        x = x[None]
+    b, n, d = x.shape
+    lib = _native.lib()
+    ws_bytes = lib.hdb_workspace_bytes(b, n, d, code(precision), int(ms), int(flags))
+    _native.check(0 if ws_bytes else -1)
+    xd = dev(x)
+    ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
+    labels = t.full((b, n), -7, dtype=t.int32, device="cuda")
+    status = t.zeros(1, dtype=t.int32, device="cuda")
+    ei = t.zeros((b, 10, n), dtype=t.int32, device="cuda")
+    er = t.zeros((b, 4, n), dtype=xd.dtype, device="cuda")
+    _native.check(lib.hdb_cluster_export(stream(), ptr(xd), b, n, d, code(precision), int(mcs), int(ms),
+                                         float(eps), -100000 if depth is None else int(depth), ptr(labels),
+                                         ptr(ws), ws_bytes, ptr(status), int(flags), ptr(ei), ptr(er)))
+    return host(labels), int(host(status)[0]), unpack_exports(host(ei), host(er), n)
+
+
+def unpack_exports(hi, hr, n):
+    exports = []
+    for w in range(hi.shape[0]):
+        e = {k: hi[w, i] for i, k in enumerate(EXPORT_INTS)}
+        e.update({k: hr[w, i] for i, k in enumerate(EXPORT_REALS)})
+        nl = int(e["meta"][0])
+        e["next_label"] = nl
+        for k in ("edge_u", "edge_v", "order", "left", "right", "sizes", "edge_w", "distance"):
+            e[k] = e[k][: n - 1]
+        for k in ("parent", "selected", "birth", "stability"):
+            e[k] = e[k][: nl - n]
+        exports.append(e)
+    return exports
+
+
 EXPORT_INTS = ("edge_u", "edge_v", "order", "left", "right", "sizes", "parent", "selected", "point_parent", "meta")
 EXPORT_REALS = ("edge_w", "distance", "birth", "stability")
 
@@ -130,17 +167,5 @@ def cluster_matrix(dist, core_dist, mcs, eps, precision, depth=None, flags=0, ex
                                          float(eps), -100000 if depth is None else int(depth), ptr(labels),
                                          ptr(ws), ws_bytes, ptr(status), int(flags), ptr(ei), ptr(er)))
     out = host(labels)
-    exports = []
-    if export:
-        hi, hr = host(ei), host(er)
-        for w in range(b):
-            e = {k: hi[w, i] for i, k in enumerate(EXPORT_INTS)}
-            e.update({k: hr[w, i] for i, k in enumerate(EXPORT_REALS)})
-            nl = int(e["meta"][0])
-            e["next_label"] = nl
-            for k in ("edge_u", "edge_v", "order", "left", "right", "sizes", "edge_w", "distance"):
-                e[k] = e[k][: n - 1]
-            for k in ("parent", "selected", "birth", "stability"):
-                e[k] = e[k][: nl - n]
-            exports.append(e)
+    exports = unpack_exports(host(ei), host(er), n) if export else []
     return out, int(host(status)[0]), exports

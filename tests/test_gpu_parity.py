@@ -192,7 +192,7 @@ def test_workload_windows(index):
     assert same_partition(got, sk_labels)
 
 
-@pytest.mark.parametrize("flags", [0, 1, 2, 4, 16, 2 | 1, 1 << 8, 2 << 8, 4 << 8])
+@pytest.mark.parametrize("flags", [0, 1, 2, 4, 16, 32, 2 | 1, 1 << 8, 2 << 8, 4 << 8, 32 | 4 << 8])
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_path_variants_agree(flags, precision):
     rng = np.random.default_rng(11)
@@ -214,3 +214,67 @@ def test_status_flags():
     assert G.cluster(x, 2, 2, 0.0, "float32")[1] & 2
     big = np.finfo(np.float64).max / 4
     assert G.cluster(np.array([[-big], [-big], [big], [big]]), 2, 5 - 3, 0.0, "float64")[1] == 1
+
+
+def oracle_window(x, mcs, ms, eps, precision):
+    """Oracle stages of one window: Prim edges and the tree intermediates."""
+    dist = oracle.distances(x, precision)
+    core = oracle.core(dist, ms, precision) if ms > 2 else None
+    u, v, w = oracle.prim(dist, core, unit_core=ms <= 2, precision=precision)
+    return (u, v, w), oracle.tree(u, v, w, mcs, eps, precision=precision)
+
+
+def prefilter_windows(precision):
+    """Windows that stress the certified distance prefilter: separations tiny
+    against the norms (cancellation), ties, duplicates, norms just inside and
+    beyond the limit under which the prefilter is enabled."""
+    rng = np.random.default_rng(77)
+    base = rng.normal(size=(256, 6))
+    grid = rng.integers(-3, 4, (256, 6)) / 4.0
+    out = {
+        "offset-1e3": base + 1e3,
+        "offset-1e5": base * 1e-3 + 1e5,
+        "grid-ties": grid,
+        "grid-offset": grid + 2 ** 20,
+        "duplicates": np.repeat(base[:64], 4, axis=0),
+        "huge-in": base * 2.0 ** 40,
+        "huge-out": base * 2.0 ** 52,
+    }
+    if precision == "float64":
+        out["huge-in"] = base * 2.0 ** 400
+        out["huge-out"] = base * 2.0 ** 460
+    return {k: v.astype(precision) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("mcs,ms,eps", [(2, 2, 0.0), (5, 5, 0.0), (4, 3, 0.1)])
+@pytest.mark.parametrize("vpt", [0, 1, 4])
+@pytest.mark.parametrize("prefilter", [0, 32])
+def test_points_path_edges_bitwise(precision, mcs, ms, eps, vpt, prefilter):
+    for name, x in prefilter_windows(precision).items():
+        labels, status, ex = G.cluster_export(x, mcs, ms, eps, precision, flags=(vpt << 8) | prefilter)
+        assert status == 0, name
+        (u, v, w), t = oracle_window(x, mcs, ms, eps, precision)
+        np.testing.assert_array_equal(ex[0]["edge_u"], u, err_msg=name)
+        np.testing.assert_array_equal(ex[0]["edge_v"], v, err_msg=name)
+        np.testing.assert_array_equal(ex[0]["edge_w"].view(BITS[precision]), w.view(BITS[precision]), err_msg=name)
+        for k in ("order", "left", "right", "sizes", "parent", "selected", "point_parent"):
+            np.testing.assert_array_equal(ex[0][k], t[k], err_msg=f"{name}:{k}")
+        for k in ("birth", "stability"):
+            np.testing.assert_array_equal(ex[0][k].view(BITS[precision]), t[k].view(BITS[precision]),
+                                          err_msg=f"{name}:{k}")
+        np.testing.assert_array_equal(labels[0], t["labels"], err_msg=name)
+
+
+@pytest.mark.parametrize("index", range(len(WORKLOADS)), ids=[w[0] for w in WORKLOADS])
+def test_workload_edges_bitwise(index):
+    name, x, precision, mcs, ms, eps, _ = WORKLOADS[index]
+    if x.shape[0] * x.shape[0] * x.shape[1] > 2048 * 2048 * 32:
+        pytest.skip("oracle stage export too slow for this size")
+    labels, status, ex = G.cluster_export(x, mcs, ms, eps, precision)
+    assert status == 0
+    (u, v, w), t = oracle_window(x, mcs, ms, eps, precision)
+    np.testing.assert_array_equal(ex[0]["edge_u"], u)
+    np.testing.assert_array_equal(ex[0]["edge_v"], v)
+    np.testing.assert_array_equal(ex[0]["edge_w"], w)
+    np.testing.assert_array_equal(labels[0], t["labels"])
