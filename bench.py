@@ -81,7 +81,7 @@ def cpu_oracle_baseline(x, mcs, ms, eps, precision, threads, budget_s=10.0):
     t0 = time.perf_counter()
     oracle.hdbscan(x[0], mcs, ms, eps, precision=precision)
     per = time.perf_counter() - t0
-    count = int(max(threads, min(len(x), budget_s / max(per, 1e-6) * threads)))
+    count = int(max(threads, budget_s / max(per, 1e-6) * threads))
     sample = [x[i % len(x)] for i in range(count)]
 
     def one(w):
@@ -108,19 +108,27 @@ def _sk_fit(args):
                    allow_single_cluster=False, alpha=1.0, n_jobs=1).fit_predict(w).max()
 
 
-def sklearn_baseline(x, mcs, ms, eps, workers, per_worker=3):
-    """scikit-learn windows/s with one single-threaded process per core."""
+def sklearn_baseline(x, mcs, ms, eps, workers, budget_s=10.0):
+    """scikit-learn windows/s with one single-threaded process per core,
+    bounded to about budget_s seconds of timed work."""
     import multiprocessing as mp
 
     for var in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
         os.environ[var] = "1"
     ctx = mp.get_context("spawn")
-    count = workers * per_worker
-    jobs = [(x[i % len(x)].astype(np.float64), mcs, ms, eps) for i in range(count)]
+
+    def jobs(count):
+        return [(x[i % len(x)].astype(np.float64), mcs, ms, eps) for i in range(count)]
+
     with cf.ProcessPoolExecutor(workers, mp_context=ctx, initializer=_sk_worker_init) as pool:
-        list(pool.map(_sk_fit, jobs[:workers]))  # start-up and import barrier
+        list(pool.map(_sk_fit, jobs(workers)))  # start-up and import barrier
         t0 = time.perf_counter()
-        list(pool.map(_sk_fit, jobs))
+        list(pool.map(_sk_fit, jobs(4 * workers)))
+        per = (time.perf_counter() - t0) / (4 * workers)
+        count = int(max(workers, budget_s / max(per, 1e-6)))
+        work = jobs(count)
+        t0 = time.perf_counter()
+        list(pool.map(_sk_fit, work, chunksize=max(1, count // (8 * workers))))
         dt = time.perf_counter() - t0
     return count / dt, count, dt
 
@@ -156,7 +164,7 @@ def main():
         src = torch.ones(count, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads and writes count*4 bytes
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": count * 4, "launches": 3}), flush=True)
         return
