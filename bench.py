@@ -144,6 +144,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-sklearn", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain exact-distance comparison run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (3 launches) and exit; "
                          "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -181,53 +182,65 @@ def main():
     xd = to_device(x, torch)
     threads = min(16, host_cores())
 
-    # Label-exact gate on every window of the batch.
-    labels, status, workspace = _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=args.flags)
-    torch.cuda.synchronize()
-    _rocm.raise_status(int(status.item()))
-    got = to_host(labels, torch)
+    def measure(flags, label):
+        """Parity gate on every window, then W warmup and K timed steps, then
+        per-stage HIP-event durations (events on the launch stream)."""
+        labels, status, workspace = _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=flags)
+        torch.cuda.synchronize()
+        _rocm.raise_status(int(status.item()))
+        got = to_host(labels, torch)
+        exact = int(sum(np.array_equal(a, b) for a, b in zip(got, expected)))
+        if exact != batch:
+            log(f"PARITY FAILURE ({label}): {batch - exact} of {batch} windows differ from the oracle")
+            sys.exit(3)
+        log(f"parity gate ({label}): {exact}/{batch} windows label-exact")
+        out = labels
+        for _ in range(args.warmup):
+            _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=flags, out=out, workspace=workspace,
+                                status=status)
+        status.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=flags, out=out, workspace=workspace,
+                                status=status)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        _rocm.raise_status(int(status.item()))
+        log(f"timed {args.steps} steps ({label}): {elapsed / args.steps * 1e3:.3f} ms/step, "
+            f"{batch * args.steps / elapsed:.0f} windows/s")
+        lib = _native.lib()
+        lib.hdb_profile(1)
+        stage_sum = {}
+        for _ in range(args.profile_steps):
+            _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=flags, out=out, workspace=workspace,
+                                status=status)
+            torch.cuda.synchronize()
+            for name, t in _native.stage_times():
+                stage_sum[name] = stage_sum.get(name, 0.0) + t
+        lib.hdb_profile(0)
+        stages = {k: v / args.profile_steps for k, v in stage_sum.items()}
+        log(f"stage ms ({label}): " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+        return batch * args.steps / elapsed, elapsed / args.steps * 1e3, stages, exact
+
     log(f"parity gate: oracle on {batch} windows with {threads} threads")
     expected = oracle_labels(x, mcs, ms, eps, precision, threads)
-    exact = int(sum(np.array_equal(a, b) for a, b in zip(got, expected)))
-    if exact != batch:
-        log(f"PARITY FAILURE: {batch - exact} of {batch} windows differ from the oracle")
-        sys.exit(3)
-    log(f"parity gate: {exact}/{batch} windows label-exact")
-
-    out = labels
-    for _ in range(args.warmup):
-        _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=args.flags, out=out, workspace=workspace,
-                            status=status)
-    status.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=args.flags, out=out, workspace=workspace,
-                            status=status)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    _rocm.raise_status(int(status.item()))
-    ms_per_step = elapsed / args.steps * 1e3
-    value = batch * args.steps / elapsed
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.0f} windows/s")
-
-    # Stage durations from HIP events recorded by the library on its stream.
-    lib = _native.lib()
-    lib.hdb_profile(1)
-    stage_sum = {}
-    for _ in range(args.profile_steps):
-        _rocm.cluster_async(xd, mcs, ms, eps, precision, flags=args.flags, out=out, workspace=workspace,
-                            status=status)
-        torch.cuda.synchronize()
-        for name, t in _native.stage_times():
-            stage_sum[name] = stage_sum.get(name, 0.0) + t
-    lib.hdb_profile(0)
-    stages = {k: v / args.profile_steps for k, v in stage_sum.items()}
-    log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+    value, ms_per_step, stages, exact = measure(args.flags, "MFMA filter" if not args.flags & 64 else "plain")
+    plain = None
+    if not args.no_plain and not args.flags & 64:
+        pv, pms, pstages, _ = measure(args.flags | 64, "plain")
+        plain = {"value": pv, "ms_per_step": pms, "stage_ms": pstages, "flags": args.flags | 64,
+                 "mfma_filter_speedup": value / pv}
     dominant = max(stages, key=stages.get)
     pairs = n * (n - 1) / 2
     dist_flops = 3.0 * d * pairs * batch  # sub, mul, add per coordinate per unordered pair
-    if dominant in ("prim_kernel", "core_topk_kernel", "distance_kernel"):
+    if dominant == "core_gram_kernel":
+        gram_flops = 2.0 * d * n * n * batch
+        achieved = gram_flops / (stages[dominant] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": dominant,
+                    "algorithmic": f"{gram_flops:.4g} FLOP per launch (Gram 2*D*N^2 per window)"}
+    elif dominant in ("prim_kernel", "core_topk_kernel", "distance_kernel"):
         peak = FP32_PEAK_TFLOPS if precision == "float32" else FP64_PEAK_TFLOPS
         achieved = dist_flops / (stages[dominant] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -298,11 +311,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "batch": batch, "n": n, "d": d,
                    "min_cluster_size": mcs, "min_samples": ms, "cluster_selection_epsilon": eps,
-                   "path": "plain exact-distance", "flags": args.flags},
+                   "path": "plain exact-distance" if args.flags & 64 else "MFMA Gram filter",
+                   "flags": args.flags},
         "label_exact_windows": exact,
         "public_api_windows_per_s": api_value,
         "public_api_calls_per_batch": len(chunks),
         "stage_ms": stages,
+        "plain": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "sklearn_baseline": sk,

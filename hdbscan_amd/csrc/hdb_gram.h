@@ -1,0 +1,461 @@
+// Core distances through a certified MFMA Gram filter (float32 windows).
+//
+// The core distance of row i is the ms-th smallest exact distance of the row,
+// self included (reference hdbscan_gpu/_cuda.py:114-118 partitions the
+// materialised matrix).  Exact means the reference's own arithmetic:
+// s = sum over k ascending of (x_ik - x_jk)^2 in float32, no contraction,
+// then the correctly rounded square root (reference hdbscan_gpu/_kernels.py:
+// 45-73).  sqrt is monotone, so the ms-th smallest distance is
+// sqrt(ms-th smallest s).
+//
+// Filter.  Matrix cores (v_mfma_f32_16x16x32_bf16) evaluate, for every pair,
+//   g = |x_i|^2 (1 +- c) + |x_j|^2 (1 +- c) +- eabs - 2 x_i . x_j
+// with x split into bf16 pieces, x = hi + lo + rest (|rest| <= 2^-16 |x|), and
+// the three products hi.hi + hi.lo + lo.hi accumulated in float32 on top of
+// the two norm terms (the accumulator's initial value).  With "+" the result
+// is an upper bound ub >= s, with "-" a lower bound lb <= s.  gram_prep_kernel
+// writes the pieces of x and of -2 x and the norm terms once per point.
+//   Pass A: U_i = ms-th smallest ub of row i, taken over the per-lane
+//           minima (each of the 16 lanes of a row keeps its M smallest ub),
+//           a subset of the row, so U_i >= the ms-th smallest ub >= t_i,
+//           the exact statistic (ms columns have s <= ub).
+//   Pass B: every column with lb <= U_i is written to the row's candidate
+//           list.  Each column with s <= t_i has lb <= s <= t_i <= U_i, so
+//           the ms smallest exact values of the row are all candidates.
+// core_exact_kernel then recomputes the candidates exactly (one lane per
+// pair, coordinate order) and selects the ms-th smallest.  A row with more
+// candidates than list slots (ties, or the filter off) is recomputed
+// exactly over all columns.
+//
+// Error bound (u = 2^-24, D = padded dimension, S = |x_i|^2 + |x_j|^2,
+// r = real squared distance <= 2 S):
+//   reference sum          |s - r| <= gamma_{D+2} r
+//   float32 squared norms  |n - |x|^2| <= gamma_D |x|^2
+//   bf16 split residual    <= 3.02 2^-16 |x_k| |y_k| per coordinate product
+//   float32 accumulation   <= 2u per multiply-add on magnitudes <= 2.1 S
+//                          (allowing any internal order and rounding),
+//                          one more rounding for the initial value
+// so c = (16 D + 48) u + 7 2^-16 bounds all of them relative to S.  Any flush
+// of subnormal bf16 operands or products stays below 2^-110 (1 + S), covered
+// by eabs = 2^-109 and 2^-110 added to c.  The filter is used only when every
+// squared norm of the window is <= 2^100, where no pair can overflow;
+// otherwise every row takes the exact path (which also checks overflow).
+#pragma once
+
+#include "hdb_common.h"
+
+namespace hdb {
+
+typedef float gram_f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 gram_bf16x8 __attribute__((ext_vector_type(8)));
+typedef short gram_s16x8 __attribute__((ext_vector_type(8)));
+
+// bf16 bits of v rounded to nearest even (finite v).
+__device__ __forceinline__ uint32_t bf16_bits(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_value(uint32_t bits) { return __uint_as_float(bits << 16); }
+
+// Minimum over the 16 lanes of a DPP row (row_ror 1, 2, 4, 8).
+__device__ __forceinline__ float row16_min(float v) {
+  v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xF, 0xF, false)));
+  v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xF, 0xF, false)));
+  v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xF, 0xF, false)));
+  v = fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false)));
+  return v;
+}
+
+// Ascending register list of the K smallest values offered.
+template <int K>
+__device__ __forceinline__ void list_offer(float* lst, float v) {
+  if (v < lst[K - 1]) {
+#pragma unroll
+    for (int q = K - 1; q > 0; --q) lst[q] = fminf(lst[q], fmaxf(lst[q - 1], v));
+    lst[0] = fminf(lst[0], v);
+  }
+}
+
+// The kth smallest (1-based) of the union of the sorted lists of the 16
+// lanes of each DPP row; every lane of the row returns it.  Equal heads are
+// removed together, so the value at which the running count first reaches
+// kth is the kth smallest.  Uniform trip count: all lanes active.
+template <int K>
+__device__ __forceinline__ float group16_kth(float* lst, int kth, int lane) {
+  int cnt = 0;
+  float ans = INFINITY;
+  bool done = false;
+  for (int it = 0; it < kth; ++it) {
+    const float h = lst[0];
+    const float m = row16_min(h);
+    const bool eq = h == m;
+    const uint64_t bal = __ballot(eq);
+    const int c = __popcll((bal >> (lane & 48)) & 0xFFFFull);
+    if (!done) {
+      cnt += c;
+      if (cnt >= kth) {
+        ans = m;
+        done = true;
+      }
+    }
+    if (eq) {
+#pragma unroll
+      for (int q = 0; q < K - 1; ++q) lst[q] = lst[q + 1];
+      lst[K - 1] = INFINITY;
+    }
+  }
+  return ans;
+}
+
+// Launch shape of the Gram candidate kernel for padded dimension DK and
+// RSET 16-row sets per wave (each B operand read from LDS feeds RSET MFMAs).
+template <int DK, int RSET> struct GramShape {
+  static constexpr int KP = (3 * DK + 31) / 32 * 32;    // three bf16 segments, MFMA K = 32
+  static constexpr int NQ = KP / 32;                    // MFMAs per 16x16 tile
+  static constexpr int KS = KP + 8;                     // LDS column stride (bf16)
+  static constexpr int TJ = DK >= 128 ? 32 : 64;        // columns per chunk
+  static constexpr int RS = RSET;
+  static constexpr int WR = 16 * RS;                    // rows per wave
+  static constexpr int MAXT = DK >= 64 ? 256 : 512;     // threads per workgroup
+  static constexpr int VC = TJ * KP / 8;                // 16-byte vectors per chunk
+  static constexpr int PF = (VC + MAXT - 1) / MAXT;     // prefetched vectors per thread
+};
+
+// Candidate slots per row (columns beyond it send the row to the exact path).
+constexpr int kGramCap = 32;
+
+// Bound constant c and half the absolute term for padded dimension dk.
+__host__ __device__ constexpr float gram_c(int dk) {
+  return float(16 * dk + 48) * 5.9604644775390625e-08f + 7.0f * 1.52587890625e-05f + 7.7037197775489434e-34f;
+}
+constexpr float kGramEh = 7.7037197775489434e-34f;  // 2^-110
+
+// Per point p of the batch: pa[p] = bf16 (hi(x) | lo(x)), pb[p] = (hi(-2x) |
+// lo(-2x)), each DK wide (zero padded); nb[p] = |x|^2 (1 + c) + eh,
+// nb[P + p] = |x|^2 (1 - c) - eh, nb[2P + p] = |x|^2.
+template <int DK>
+__global__ __launch_bounds__(256) void gram_prep_kernel(const float* x, int64_t points, int d, uint16_t* pa,
+                                                        uint16_t* pb, float* nb) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= points) return;
+  const float* xp = x + (size_t)p * d;
+  uint16_t* ap = pa + (size_t)p * 2 * DK;
+  uint16_t* bp = pb + (size_t)p * 2 * DK;
+  const float c = gram_c(DK);
+  float sq = 0.f;
+  for (int k0 = 0; k0 < DK; k0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = k0 + e < d ? xp[k0 + e] : 0.f;
+    gram_s16x8 ah, al, bh, bl;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      sq = fmaf(v[e], v[e], sq);
+      const uint32_t h = bf16_bits(v[e]);
+      ah[e] = (short)h;
+      al[e] = (short)bf16_bits(v[e] - bf16_value(h));
+      const float w = -2.f * v[e];
+      const uint32_t g = bf16_bits(w);
+      bh[e] = (short)g;
+      bl[e] = (short)bf16_bits(w - bf16_value(g));
+    }
+    *reinterpret_cast<gram_s16x8*>(ap + k0) = ah;
+    *reinterpret_cast<gram_s16x8*>(ap + DK + k0) = al;
+    *reinterpret_cast<gram_s16x8*>(bp + k0) = bh;
+    *reinterpret_cast<gram_s16x8*>(bp + DK + k0) = bl;
+  }
+  nb[p] = fmaf(sq, c, sq) + kGramEh;
+  nb[points + p] = fmaf(sq, -c, sq) - kGramEh;
+  nb[2 * points + p] = sq;
+}
+
+// LDS bytes of core_gram_kernel<.., DK, RSET> for a window of n points.
+template <int DK, int RSET>
+HDB_HD uint32_t gram_lds_bytes(int n) {
+  using G = GramShape<DK, RSET>;
+  return (uint32_t)G::TJ * G::KS * 2u + 2u * (uint32_t)n * 4u;
+}
+
+template <int DK> constexpr int gram_max_threads() { return GramShape<DK, 1>::MAXT; }
+
+// Source offset (in bf16) inside a point's (hi | lo) pair of the 8 operand
+// entries starting at K index k, for the segment order [s0 | s1 | s2];
+// -1: zero padding.
+template <int DK>
+__device__ __forceinline__ int gram_src(int k, int lo_segment) {
+  const int seg = k / DK, m = k - seg * DK;
+  return seg >= 3 ? -1 : (seg == lo_segment ? DK : 0) + m;
+}
+
+// Pass A and B of the filter.  One workgroup: WR rows per wave (RS sets of
+// 16, each with its MFMA A operand in registers); columns streamed through
+// LDS in chunks of TJ points, twice; the next chunk's loads are in flight
+// while the current chunk is consumed.  Output: cand[b][i][kGramCap] column
+// indices and ncand[b][i] (> kGramCap: recompute the row exactly).
+// Grid: (ceil(n / rows), batch).  M: per-lane minima kept in pass A (1, 2).
+template <int M, int DK, int RSET>
+__global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(int n, int ms, const uint16_t* pa,
+                                                                          const uint16_t* pb, const float* nb,
+                                                                          int64_t points, uint16_t* cand,
+                                                                          int32_t* ncand) {
+  using G = GramShape<DK, RSET>;
+  constexpr int NQ = G::NQ, KS = G::KS, TJ = G::TJ, PF = G::PF, VC = G::VC;
+  constexpr int RS = G::RS, WR = G::WR, CAP = kGramCap;
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ uint32_t sh_nmax;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
+  const int row0 = blockIdx.x * W * WR;
+  uint16_t* bcol = reinterpret_cast<uint16_t*>(lds);          // [TJ][KS] bf16
+  float* nbound = reinterpret_cast<float*>(bcol + TJ * KS);   // [2][n]
+  const size_t rbase = (size_t)b * n;
+
+  if (tid == 0) sh_nmax = 0;
+  // Zero padding of the column operands (never overwritten).
+  for (int idx = tid; idx < TJ * KS; idx += T) {
+    const int k = idx % KS;
+    if (k >= 3 * DK) bcol[idx] = 0;
+  }
+  __syncthreads();  // sh_nmax initialised before the atomics below
+  uint32_t nmax = 0;
+  for (int j = tid; j < n; j += T) {
+    nbound[j] = nb[rbase + j];
+    nbound[n + j] = nb[points + rbase + j];
+    const float sq = nb[2 * points + rbase + j];
+    // NaN compares false below, so it disables the filter like a huge norm.
+    nmax = max(nmax, sq <= 1.2676506002282294e30f ? __float_as_uint(sq) : 0xFFFFFFFFu);
+  }
+  atomicMax(&sh_nmax, nmax);
+  __syncthreads();
+  if (sh_nmax > __float_as_uint(1.2676506002282294e30f)) {
+    // Filter off for this window: every row goes to the exact path.
+    for (int r = tid; r < W * WR; r += T)
+      if (row0 + r < n) ncand[rbase + row0 + r] = CAP + 1;
+    return;
+  }
+
+  // Accumulator layout: register r holds row 4 (lane >> 4) + r of the set,
+  // column lane & 15.
+  const int grp = lane >> 4;
+  const int nchunks = (n + TJ - 1) / TJ;
+  const uint32_t below = (1u << (lane & 15)) - 1u;
+
+  // Operand A of set g: row g*16 + (lane & 15) of the wave's strip as
+  // [hi(x) | hi(x) | lo(x)].
+  gram_bf16x8 av[RS][NQ];
+#pragma unroll
+  for (int g = 0; g < RS; ++g) {
+    const int ai = min(row0 + wave * WR + g * 16 + (lane & 15), n - 1);
+    const uint16_t* ap = pa + (rbase + ai) * 2 * DK;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int src = gram_src<DK>(32 * q + 8 * grp, 2);
+      av[g][q] = src < 0 ? gram_bf16x8{} : *reinterpret_cast<const gram_bf16x8*>(ap + src);
+    }
+  }
+  float U[RS][4];
+  int cnt[RS][4];
+  float mn[RS][4][M];
+#pragma unroll
+  for (int g = 0; g < RS; ++g)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cnt[g][r] = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    float nrow[RS][4];
+#pragma unroll
+    for (int g = 0; g < RS; ++g)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = min(row0 + wave * WR + g * 16 + 4 * grp + r, n - 1);
+        nrow[g][r] = nbound[pass * n + i];
+#pragma unroll
+        for (int q = 0; q < M; ++q) mn[g][r][q] = INFINITY;
+      }
+    gram_s16x8 pre[PF];
+    // Column chunk: operand vectors [hi(-2x) | lo(-2x) | hi(-2x)], copied.
+    auto fetch = [&](int c0) {
+#pragma unroll
+      for (int e = 0; e < PF; ++e) {
+        const int v = tid + e * T;
+        const int c = v / (G::KP / 8), k = (v - c * (G::KP / 8)) * 8, j = c0 + c;
+        const int src = gram_src<DK>(k, 1);
+        pre[e] = (v < VC && j < n && src >= 0)
+                     ? *reinterpret_cast<const gram_s16x8*>(pb + (rbase + j) * 2 * DK + src)
+                     : gram_s16x8{};
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int e = 0; e < PF; ++e) {
+        const int v = tid + e * T;
+        const int c = v / (G::KP / 8), k = (v - c * (G::KP / 8)) * 8;
+        if (v < VC) *reinterpret_cast<gram_s16x8*>(bcol + c * KS + k) = pre[e];
+      }
+    };
+    // Threads beyond the prefetch width (small workgroups) load directly.
+    auto direct = [&](int c0) {
+      for (int v = tid + PF * T; v < VC; v += T) {
+        const int c = v / (G::KP / 8), k = (v - c * (G::KP / 8)) * 8, j = c0 + c;
+        const int src = gram_src<DK>(k, 1);
+        *reinterpret_cast<gram_s16x8*>(bcol + c * KS + k) =
+            (j < n && src >= 0) ? *reinterpret_cast<const gram_s16x8*>(pb + (rbase + j) * 2 * DK + src)
+                                : gram_s16x8{};
+      }
+    };
+    __syncthreads();
+    fetch(0);
+    store();
+    direct(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ++ch) {
+      const int c0 = ch * TJ;
+      const bool more = ch + 1 < nchunks;
+      if (more) fetch(c0 + TJ);
+      const int cn = min(TJ, n - c0);
+      for (int s0 = 0; s0 < cn; s0 += 16) {
+        const int cl = s0 + (lane & 15);
+        const int j = c0 + cl;
+        const float ncj = j < n ? nbound[pass * n + j] : INFINITY;
+        gram_f32x4 acc[RS];
+#pragma unroll
+        for (int g = 0; g < RS; ++g)
+          acc[g] = gram_f32x4{nrow[g][0] + ncj, nrow[g][1] + ncj, nrow[g][2] + ncj, nrow[g][3] + ncj};
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const gram_bf16x8 bv = *reinterpret_cast<const gram_bf16x8*>(bcol + cl * KS + 32 * q + 8 * grp);
+#pragma unroll
+          for (int g = 0; g < RS; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[g][q], bv, acc[g], 0, 0, 0);
+        }
+        if (pass == 0) {
+#pragma unroll
+          for (int g = 0; g < RS; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              if constexpr (M == 1) {
+                mn[g][r][0] = fminf(mn[g][r][0], acc[g][r]);
+              } else {
+                mn[g][r][1] = fminf(mn[g][r][1], fmaxf(mn[g][r][0], acc[g][r]));
+                mn[g][r][0] = fminf(mn[g][r][0], acc[g][r]);
+              }
+            }
+        } else {
+          // Candidates of row (set g, 4 grp + r): the 16 lanes of a group
+          // hold its 16 columns; slots follow the lane order.
+#pragma unroll
+          for (int g = 0; g < RS; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const bool c = !(acc[g][r] > U[g][r]);
+              const uint64_t mask = __ballot(c);
+              if (mask) {
+                const uint32_t m16 = (uint32_t)(mask >> (16 * grp)) & 0xFFFFu;
+                const int pos = cnt[g][r] + __popc(m16 & below);
+                const int i = row0 + wave * WR + g * 16 + 4 * grp + r;
+                if (c && pos < CAP && i < n) cand[(rbase + i) * CAP + pos] = (uint16_t)j;
+                cnt[g][r] += __popc(m16);
+              }
+            }
+        }
+      }
+      __syncthreads();
+      if (more) {
+        store();
+        direct(c0 + TJ);
+        __syncthreads();
+      }
+    }
+    if (pass == 0) {
+#pragma unroll
+      for (int g = 0; g < RS; ++g)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) U[g][r] = group16_kth<M>(mn[g][r], ms, lane);
+    }
+  }
+  if ((lane & 15) == 0) {
+#pragma unroll
+    for (int g = 0; g < RS; ++g)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + wave * WR + g * 16 + 4 * grp + r;
+        if (i < n) ncand[rbase + i] = cnt[g][r];
+      }
+  }
+}
+
+// Exact coordinate-order squared distance of rows xi and xj (d floats), 16
+// loads in flight per operand; 16-byte loads when both rows are aligned.
+__device__ __forceinline__ float exact_sq(const float* xi, const float* xj, int d) {
+  float t = 0.f;
+  const bool vec = ((d & 3) == 0) && ((((uintptr_t)xi) | ((uintptr_t)xj)) & 15) == 0;
+  for (int k0 = 0; k0 < d; k0 += 16) {
+    float a[16], c[16];
+    if (vec && k0 + 16 <= d) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 va = reinterpret_cast<const float4*>(xi + k0)[q];
+        const float4 vc = reinterpret_cast<const float4*>(xj + k0)[q];
+        a[4 * q] = va.x; a[4 * q + 1] = va.y; a[4 * q + 2] = va.z; a[4 * q + 3] = va.w;
+        c[4 * q] = vc.x; c[4 * q + 1] = vc.y; c[4 * q + 2] = vc.z; c[4 * q + 3] = vc.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        a[k] = k0 + k < d ? xi[k0 + k] : 0.f;
+        c[k] = k0 + k < d ? xj[k0 + k] : 0.f;
+      }
+    }
+    // Zero padding adds (0 - 0)^2 = +0, which leaves the sum unchanged.
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const float dl = a[k] - c[k];
+      t = t + dl * dl;
+    }
+  }
+  return t;
+}
+
+// Core distance of each row from its candidate list: one thread per row
+// recomputes its candidates exactly (or reads them from the exact distance
+// matrix when one is materialised) and keeps the KMAX smallest; a row
+// marked for the exact path streams every column instead.
+template <int KMAX>
+__global__ __launch_bounds__(256) void core_exact_kernel(const float* x, const float* dist, int64_t rows, int n,
+                                                         int d, int ms, const uint16_t* cand, const int32_t* ncand,
+                                                         float* core, int32_t* status) {
+  constexpr int CAP = kGramCap;
+  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int bad = 0;
+  if (row < rows) {
+    const int64_t b = row / n;
+    const float* xb = x + (size_t)b * n * d;
+    const float* xi = xb + (size_t)(row - b * n) * d;
+    const float* di = dist ? dist + (size_t)row * n : nullptr;
+    const int m = ncand[row];
+    float lst[KMAX];
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) lst[q] = INFINITY;
+    if (m <= CAP) {
+      for (int p = 0; p < m; ++p) {
+        const int j = cand[row * CAP + p];
+        // The matrix holds sqrt(s); squares of its values order like s.
+        const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
+        list_offer<KMAX>(lst, t);
+      }
+    } else {
+      for (int j = 0; j < n; ++j) {
+        const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
+        bad |= !isfinite(t);
+        list_offer<KMAX>(lst, t);
+      }
+    }
+    float t = lst[0];
+#pragma unroll
+    for (int q = 1; q < KMAX; ++q)
+      if (q == ms - 1) t = lst[q];
+    core[row] = di ? t : sqrt(t);
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+}
+
+}  // namespace hdb

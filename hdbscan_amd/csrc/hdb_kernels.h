@@ -15,18 +15,29 @@ __global__ __launch_bounds__(256) void check_finite_kernel(const Real* x, size_t
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 2);
 }
 
-// Dense distances out[b][i][j] for a 64x64 tile per workgroup (4x4 per
-// thread).  Coordinates are staged through LDS in chunks of 16 dimensions;
-// each output sums (x_ik - x_jk)^2 for k ascending, then takes the correctly
-// rounded square root, so out[i][j] and out[j][i] are bitwise equal to the
-// reference's triangle-and-mirror kernel (reference hdbscan_gpu/_kernels.py:45-73).
+// Dense distances out[b][i][j], 64x64 tiles (4x4 per thread).  Only tiles
+// on or above the diagonal are computed; each is written as is and, through
+// an LDS transpose, to its mirror position.  Coordinates are staged through
+// LDS in chunks of 16 dimensions; each output sums (x_ik - x_jk)^2 for k
+// ascending, then takes the correctly rounded square root.  x_i - x_j is the
+// exact negation of x_j - x_i, so the mirror holds the same bits the
+// reference's triangle-and-mirror kernel writes (reference
+// hdbscan_gpu/_kernels.py:45-73).  Grid: (nt (nt + 1) / 2, batch) for nt
+// tiles per side; block t is the t-th tile of the upper triangle in row order.
 template <typename Real>
 __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int d, Real* out, int32_t* status) {
   constexpr int TILE = 64, KC = 16;
   __shared__ Real si[KC][TILE];
   __shared__ Real sj[KC][TILE];
-  const int b = blockIdx.z;
-  const int ti = blockIdx.y * TILE, tj = blockIdx.x * TILE;
+  __shared__ Real st[TILE][TILE + 1];
+  const int nt = (n + TILE - 1) / TILE;
+  int t = blockIdx.x, rt = 0;
+  while (t >= nt - rt) {
+    t -= nt - rt;
+    ++rt;
+  }
+  const int b = blockIdx.y;
+  const int ti = rt * TILE, tj = (rt + t) * TILE;
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const Real* xb = x + (size_t)b * n * d;
   Real tot[4][4];
@@ -64,15 +75,25 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int gi = ti + ty * 4 + r;
-    if (gi >= n) continue;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int gj = tj + tx * 4 + q;
-      if (gj < n) {
-        const Real v = sqrt(tot[r][q]);
+      const Real v = sqrt(tot[r][q]);
+      st[ty * 4 + r][tx * 4 + q] = v;
+      if (gi < n && gj < n) {
         bad |= !isfinite(v);
         ob[(size_t)gi * n + gj] = v;
       }
+    }
+  }
+  if (tj != ti) {
+    __syncthreads();
+    // Mirror tile: row tj + c, column ti + r holds st[r][c]; consecutive
+    // threads write consecutive columns.
+    for (int idx = tid; idx < TILE * TILE; idx += 256) {
+      const int c = idx / TILE, r = idx - c * TILE;
+      const int gi = tj + c, gj = ti + r;
+      if (gi < n && gj < n) ob[(size_t)gi * n + gj] = st[r][c];
     }
   }
   if (__any(bad) && (tid & 63) == 0) atomicOr(status, 1);

@@ -19,6 +19,7 @@
 
 #include "../../include/hdbscan_rocm.h"
 #include "hdb_common.h"
+#include "hdb_gram.h"
 #include "hdb_kernels.h"
 #include "hdb_prim.h"
 
@@ -108,13 +109,17 @@ struct Plan {
   bool core = false;      // core distances needed (min_samples > 2)
   bool core_topk = false; // core from points with a register list
   int kmax = 0, core_dk = 0;
+  bool core_gram = false; // core through the certified MFMA Gram filter
+  int gram_dk = 0, gram_rs = 0, gram_threads = 0;
+  uint32_t gram_lds = 0;
   bool tree_lds = true;
   uint32_t tree_lds_bytes = 0;
   size_t arena_stride = 0;
   size_t slot_stride = 0;
   bool need_matrix = false;
   int flags = 0;
-  size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0, off_matrix = 0, total = 0;
+  size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0, off_matrix = 0;
+  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, total = 0;
 };
 
 int pad_dims(int64_t d) {
@@ -123,6 +128,54 @@ int pad_dims(int64_t d) {
   if (d <= 32) return 32;
   if (d <= 64) return 64;
   return 0;
+}
+
+// Row sets per wave of the Gram core kernel: 2 or 4 up to 32 dimensions,
+// 1 or 2 up to 128, 1 beyond (register budget).  Flag bits 12..13 pick one
+// (1 -> 1, 2 -> 2, 3 -> 4); 0 chooses 2.
+int gram_rsets(int dk, int flags) {
+  const int want = (flags >> 12) & 3;
+  const int rs = want == 1 ? 1 : want == 3 ? 4 : 2;
+  if (dk <= 32) return rs == 1 ? 2 : rs;
+  if (dk <= 128) return rs == 4 ? 2 : rs;
+  return 1;
+}
+
+template <int DK>
+uint32_t gram_bytes_rs(int rs, int64_t n) {
+  if (rs == 4) return gram_lds_bytes<DK, 4>((int)n);
+  if (rs == 2) return gram_lds_bytes<DK, 2>((int)n);
+  return gram_lds_bytes<DK, 1>((int)n);
+}
+
+uint32_t gram_bytes(int dk, int rs, int64_t n) {
+  switch (dk) {
+    case 8: return gram_bytes_rs<8>(rs, n);
+    case 16: return gram_bytes_rs<16>(rs, n);
+    case 32: return gram_bytes_rs<32>(rs, n);
+    case 64: return gram_bytes_rs<64>(rs, n);
+    case 128: return gram_bytes_rs<128>(rs, n);
+    default: return gram_bytes_rs<256>(rs, n);
+  }
+}
+
+// Gram-filtered core distances: float32, 3 <= min_samples <= 16, at most
+// 256 dimensions.  Waves per workgroup: as many as the padding allows, fewer
+// for small windows.
+void set_gram(Plan& P, int64_t n, int64_t d, int32_t ms, uint32_t lds_limit) {
+  P.core_gram = false;
+  if (!P.core || P.rs != 4 || ms < 3 || ms > 16 || d > 256 || (P.flags & HDB_FLAG_PLAIN)) return;
+  const int dk = d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
+  const int rs = gram_rsets(dk, P.flags);
+  int waves = (dk >= 64 ? 256 : 512) / 64;
+  const int wr = 16 * rs;  // rows per wave
+  while (waves > 1 && (int64_t)(waves / 2) * wr >= n) waves /= 2;
+  if (gram_bytes(dk, rs, n) > lds_limit - 1024u) return;
+  P.core_gram = true;
+  P.gram_dk = dk;
+  P.gram_rs = rs;
+  P.gram_threads = waves * 64;
+  P.gram_lds = gram_bytes(dk, rs, n);
 }
 
 int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int flags, Plan* p) {
@@ -141,6 +194,7 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.core_topk = P.core && ms <= 64;
   P.kmax = ms <= 8 ? 8 : ms <= 16 ? 16 : ms <= 32 ? 32 : 64;
   P.core_dk = pad_dims(d);
+  set_gram(P, n, d, ms, lds_limit);
   // Points path: owned coordinates must fit 64 VGPRs per thread.
   // Vertices per Prim thread: about 512 threads per window measured best at
   // N=1024 (fewer waves to synchronise per step, two slots of ILP); larger
@@ -187,6 +241,12 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.slot_stride = align256((size_t)n * (rs == 4 ? sizeof(ClusterSlot<float>) : sizeof(ClusterSlot<double>)));
   P.off_slots = o; o = align256(o + (size_t)batch * P.slot_stride);
   P.off_matrix = o; o = align256(o + (P.need_matrix ? bn * (size_t)n * rs : 0));
+  P.off_cand = o; o = align256(o + (P.core_gram ? bn * kGramCap * 2 : 0));
+  P.off_ncand = o; o = align256(o + (P.core_gram ? bn * 4 : 0));
+  const size_t pieces = P.core_gram ? bn * 2 * (size_t)P.gram_dk * 2 : 0;
+  P.off_pa = o; o = align256(o + pieces);
+  P.off_pb = o; o = align256(o + pieces);
+  P.off_nb = o; o = align256(o + (P.core_gram ? 3 * bn * 4 : 0));
   P.total = o;
   *p = P;
   return 0;
@@ -194,7 +254,8 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
 
 template <typename Real>
 int launch_distances(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_t d, Real* out, int32_t* status) {
-  const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64), (unsigned)batch);
+  const unsigned nt = (unsigned)((n + 63) / 64);
+  const dim3 grid(nt * (nt + 1) / 2, (unsigned)batch);
   distance_kernel<Real><<<grid, 256, 0, s>>>(x, (int)n, (int)d, out, status);
   return check_launch("distance_kernel");
 }
@@ -213,10 +274,79 @@ int launch_topk_k(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_
   return check_launch("core_topk_kernel");
 }
 
+struct GramBuffers {
+  uint16_t* pa;
+  uint16_t* pb;
+  float* nb;
+  uint16_t* cand;
+  int32_t* ncand;
+};
+
+template <int M, int DK, int RSET>
+int launch_gram_k(hipStream_t s, const Plan& P, int64_t batch, int64_t n, int ms, const GramBuffers& g) {
+  auto fn = core_gram_kernel<M, DK, RSET>;
+  if (int r = allow_lds((const void*)fn, P.gram_lds)) return r;
+  const int rows = P.gram_threads / 64 * GramShape<DK, RSET>::WR;
+  const dim3 grid((unsigned)((n + rows - 1) / rows), (unsigned)batch);
+  fn<<<grid, P.gram_threads, P.gram_lds, s>>>((int)n, ms, g.pa, g.pb, g.nb, batch * n, g.cand, g.ncand);
+  return check_launch("core_gram_kernel");
+}
+
+template <int M, int DK, int RA, int RB>
+int launch_gram_r(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d, int ms,
+                  const GramBuffers& g) {
+  const int64_t points = batch * n;
+  gram_prep_kernel<DK><<<(unsigned)((points + 255) / 256), 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb);
+  if (int r = check_launch("gram_prep_kernel")) return r;
+  stage_mark(s, "gram_prep_kernel");
+  if (P.gram_rs == RB) return launch_gram_k<M, DK, RB>(s, P, batch, n, ms, g);
+  return launch_gram_k<M, DK, RA>(s, P, batch, n, ms, g);
+}
+
+template <int M>
+int launch_gram_d(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d, int ms,
+                  const GramBuffers& g) {
+  switch (P.gram_dk) {
+    case 8: return launch_gram_r<M, 8, 2, 4>(s, P, x, batch, n, d, ms, g);
+    case 16: return launch_gram_r<M, 16, 2, 4>(s, P, x, batch, n, d, ms, g);
+    case 32: return launch_gram_r<M, 32, 2, 4>(s, P, x, batch, n, d, ms, g);
+    case 64: return launch_gram_r<M, 64, 1, 2>(s, P, x, batch, n, d, ms, g);
+    case 128: return launch_gram_r<M, 128, 1, 2>(s, P, x, batch, n, d, ms, g);
+    case 256: return launch_gram_r<M, 256, 1, 1>(s, P, x, batch, n, d, ms, g);
+  }
+  return fail("internal: unsupported Gram padding");
+}
+
+// Split operands, candidates (MFMA bounds), then the exact selection.
+// Per-lane minima kept in pass A: one for min_samples <= 3, else two (the
+// row bound stays tight unless three of the nearest share a lane).
+int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist, int64_t batch, int64_t n,
+                int64_t d, int ms, float* core, unsigned char* ws, int32_t* status) {
+  GramBuffers g;
+  g.pa = reinterpret_cast<uint16_t*>(ws + P.off_pa);
+  g.pb = reinterpret_cast<uint16_t*>(ws + P.off_pb);
+  g.nb = reinterpret_cast<float*>(ws + P.off_nb);
+  g.cand = reinterpret_cast<uint16_t*>(ws + P.off_cand);
+  g.ncand = reinterpret_cast<int32_t*>(ws + P.off_ncand);
+  const int r = ms <= 3 ? launch_gram_d<1>(s, P, x, batch, n, d, ms, g) : launch_gram_d<2>(s, P, x, batch, n, d, ms, g);
+  if (r) return r;
+  stage_mark(s, "core_gram_kernel");
+  const int64_t rows = batch * n;
+  const unsigned blocks = (unsigned)((rows + 255) / 256);
+  if (ms <= 8)
+    core_exact_kernel<8><<<blocks, 256, 0, s>>>(x, dist, rows, (int)n, (int)d, ms, g.cand, g.ncand, core, status);
+  else
+    core_exact_kernel<16><<<blocks, 256, 0, s>>>(x, dist, rows, (int)n, (int)d, ms, g.cand, g.ncand, core, status);
+  return check_launch("core_exact_kernel");
+}
+
 template <typename Real>
 int launch_core(hipStream_t s, const Plan& P, const Real* x, int64_t batch, int64_t n, int64_t d, int ms,
-                Real* core, Real* matrix, int32_t* status, bool matrix_ready) {
-  if (P.core_topk) {
+                Real* core, Real* matrix, unsigned char* ws, int32_t* status, bool matrix_ready) {
+  if constexpr (sizeof(Real) == 4) {
+    if (P.core_gram) return launch_gram(s, P, x, matrix_ready ? matrix : nullptr, batch, n, d, ms, core, ws, status);
+  }
+  if (P.core_topk && !matrix_ready) {
     switch (P.kmax) {
       case 8: return launch_topk_k<Real, 8>(s, x, batch, n, d, ms, core, P.core_dk, status);
       case 16: return launch_topk_k<Real, 16>(s, x, batch, n, d, ms, core, P.core_dk, status);
@@ -306,9 +436,9 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
       stage_mark(s, "distance_kernel");
     }
     if (!unit) {
-      if (int r = launch_core<Real>(s, P, x, batch, n, d, ms, core, matrix, status, matrix_ready)) return r;
+      if (int r = launch_core<Real>(s, P, x, batch, n, d, ms, core, matrix, ws, status, matrix_ready)) return r;
       core_in = core;
-      stage_mark(s, P.core_topk ? "core_topk_kernel" : "core_select_kernel");
+      stage_mark(s, P.core_gram ? "core_exact_kernel" : P.core_topk ? "core_topk_kernel" : "core_select_kernel");
     }
   }
   PrimArgs pa;
@@ -444,22 +574,27 @@ int hdb_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t
 
 int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,
                        int32_t min_samples, void* core, void* workspace, size_t workspace_bytes,
-                       int32_t* status) {
+                       int32_t* status, int flags) {
   g_error.clear();
   if (!x || !core || !status) return fail("null pointer argument");
+  // The cluster plan with a materialised matrix lays out the matrix (for
+  // min_samples > 64) and the Gram candidate lists (3 <= min_samples <= 16).
   Plan P;
-  if (make_plan(batch, n, d, dtype, min_samples, 0, &P)) return -1;
+  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX, &P)) return -1;
   P.core = true;
   P.core_topk = min_samples <= 64;
   P.kmax = min_samples <= 8 ? 8 : min_samples <= 16 ? 16 : min_samples <= 32 ? 32 : 64;
-  const size_t need = P.core_topk ? 0 : (size_t)batch * n * n * P.rs;
+  const size_t bn = (size_t)batch * n;
+  const size_t need = P.core_gram ? P.total : P.core_topk ? 0 : P.off_matrix + bn * n * P.rs;
   if (workspace_bytes < need || (need && !workspace)) return fail("workspace too small");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
   if (dtype == HDB_F32)
     return launch_core<float>(s, P, static_cast<const float*>(x), batch, n, d, min_samples, static_cast<float*>(core),
-                              static_cast<float*>(workspace), status, false);
+                              reinterpret_cast<float*>(ws + P.off_matrix), ws, status, false);
   return launch_core<double>(s, P, static_cast<const double*>(x), batch, n, d, min_samples,
-                             static_cast<double*>(core), static_cast<double*>(workspace), status, false);
+                             static_cast<double*>(core), reinterpret_cast<double*>(ws + P.off_matrix), ws, status,
+                             false);
 }
 
 int hdb_cluster_matrix(void* stream, const void* distances, const void* core, int64_t batch, int64_t n, int dtype,

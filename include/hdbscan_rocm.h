@@ -49,7 +49,10 @@ extern "C" {
 #define HDB_FLAG_MATRIX_INPUT 8 /* sizing for hdb_cluster_matrix (distances given) */
 #define HDB_FLAG_SERIAL_SORT 16 /* one-lane introsort (the parallel one is default) */
 #define HDB_FLAG_PREFILTER 32   /* Prim: certified FMA Gram bound skips exact distances */
-/* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic. */
+#define HDB_FLAG_PLAIN 64       /* no MFMA Gram filter: every core distance exact */
+/* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic.
+ * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
+ * 3 -> 4; 0 = 2; clamped to what the dimension allows). */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int hdb_version(void);
@@ -101,12 +104,14 @@ int hdb_distances(void *stream, const void *x, int64_t batch, int64_t n,
                   int64_t d, int dtype, void *out, int32_t *status);
 
 /* Core distances core[batch][n]: the (min_samples-1)-th smallest distance of
- * each row, self included.  Needs hdb_workspace_bytes(..., HDB_FLAG_MATRIX)
- * bytes of workspace when min_samples > 64 (0 otherwise).
+ * each row, self included.  Workspace: hdb_workspace_bytes(batch, n, d,
+ * dtype, min_samples, flags | HDB_FLAG_MATRIX) bytes always suffice (none
+ * is used when min_samples <= 64 and the MFMA filter does not apply).
  * Replaces reference hdbscan_gpu/_cuda.py:216-220 (cp.partition). */
 int hdb_core_distances(void *stream, const void *x, int64_t batch, int64_t n,
                        int64_t d, int dtype, int32_t min_samples, void *core,
-                       void *workspace, size_t workspace_bytes, int32_t *status);
+                       void *workspace, size_t workspace_bytes, int32_t *status,
+                       int flags);
 
 /* Graph and tree stages from a given distance matrix [batch][n][n] and core
  * distances [batch][n] (core may be NULL when unit_core != 0, meaning every

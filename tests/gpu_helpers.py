@@ -66,7 +66,7 @@ def distances(x, precision):
     return host(out), int(host(status)[0])
 
 
-def core(x, min_samples, precision):
+def core(x, min_samples, precision, flags=0):
     t = torch()
     x = np.asarray(x, real(precision))
     if x.ndim == 2:
@@ -75,10 +75,11 @@ def core(x, min_samples, precision):
     xd = dev(x)
     out = t.empty((b, n), dtype=xd.dtype, device="cuda")
     status = t.zeros(1, dtype=t.int32, device="cuda")
-    ws_bytes = b * n * n * x.itemsize
+    ws_bytes = _native.lib().hdb_workspace_bytes(b, n, d, code(precision), int(min_samples), int(flags) | 2)
+    assert ws_bytes > 0, _native.lib().hdb_last_error()
     ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
     _native.check(_native.lib().hdb_core_distances(stream(), ptr(xd), b, n, d, code(precision), int(min_samples),
-                                                   ptr(out), ptr(ws), ws_bytes, ptr(status)))
+                                                   ptr(out), ptr(ws), ws_bytes, ptr(status), int(flags)))
     return host(out), int(host(status)[0])
 
 

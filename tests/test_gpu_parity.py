@@ -42,17 +42,49 @@ def test_distances_bitwise(precision, size, dims):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("dims", [3, 16, 40, 100])
-@pytest.mark.parametrize("ms", [3, 5, 8, 17, 64, 65, 100])
-def test_core_distances_bitwise(precision, dims, ms):
+@pytest.mark.parametrize("ms", [3, 5, 8, 16, 17, 64, 65, 100])
+@pytest.mark.parametrize("flags", [0, 64])
+def test_core_distances_bitwise(precision, dims, ms, flags):
+    """flags 0: the MFMA Gram filter where it applies (float32, ms <= 16);
+    flags 64: every distance exact."""
     rng = np.random.default_rng(ms * 7 + dims)
     n = 300
     x = rng.integers(-6, 7, (2, n, dims)).astype(precision) / 4
     x[1] = rng.normal(size=(n, dims))
-    got, status = G.core(x, ms, precision)
+    got, status = G.core(x, ms, precision, flags)
     assert status == 0
     for b in range(2):
         exp = oracle.core(oracle.distances(x[b], precision), ms, precision)
         np.testing.assert_array_equal(got[b].view(BITS[precision]), exp.view(BITS[precision]))
+
+
+def gram_core_windows():
+    """float32 windows for the Gram-filtered core distances: every shape
+    class of the launch (one wave to sixteen, ragged last chunk and strip),
+    wide dimensions, unit-norm embeddings with exact duplicates, and the
+    stress windows of the distance prefilter."""
+    rng = np.random.default_rng(4242)
+    out = {f"stress-{k}": v for k, v in prefilter_windows("float32").items()}
+    for n, d in [(3, 2), (5, 4), (17, 8), (77, 16), (1000, 16), (1024, 16), (600, 128), (333, 200), (130, 256)]:
+        out[f"normal-{n}x{d}"] = rng.normal(size=(n, d))
+    e = rng.normal(size=(700, 128))
+    e /= np.linalg.norm(e, axis=1, keepdims=True)
+    e[rng.choice(700, 21, replace=False)] = e[rng.integers(0, 700, 21)]
+    out["embed-dup"] = e
+    out["tiny-scale"] = rng.normal(size=(200, 8)) * 1e-19
+    out["subnormal"] = rng.integers(-4, 5, (100, 3)) * 2.0 ** -140
+    return {k: v.astype(np.float32) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("ms", [3, 4, 8, 9, 16])
+def test_core_gram_filter_bitwise(ms):
+    for name, x in gram_core_windows().items():
+        if ms > len(x):
+            continue
+        got, status = G.core(x, ms, "float32")
+        assert status == 0, name
+        exp = oracle.core(oracle.distances(x, "float32"), ms, "float32")
+        np.testing.assert_array_equal(got[0].view(np.uint32), exp.view(np.uint32), err_msg=name)
 
 
 def path_graph(weights, dtype):
