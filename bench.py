@@ -133,6 +133,68 @@ def sklearn_baseline(x, mcs, ms, eps, workers, budget_s=10.0):
     return count / dt, count, dt
 
 
+def bench_ragged(args, spec, torch):
+    """Windows of uneven size in one backend call (hdb_cluster_ragged): parity
+    gate on every window, then timed steps; prints one JSON line."""
+    from hdbscan_amd import _rocm, workloads
+
+    count = args.batch or spec["batch"]
+    d, precision = spec["d"], spec["precision"]
+    mcs, ms, eps = spec["min_cluster_size"], spec["min_samples"], spec["cluster_selection_epsilon"]
+    real = np.float32 if precision == "float32" else np.float64
+    windows = [w.astype(real) for w in getattr(workloads, args.workload)(seed=0, count=count)]
+    sizes = np.array([len(w) for w in windows])
+    log(f"generating {args.workload}: {count} windows, N {sizes.min()}..{sizes.max()} "
+        f"(mean {sizes.mean():.0f}), D={d} {precision}")
+    packed, offsets = _rocm.pack_windows(windows, precision)
+    threads = min(16, host_cores())
+    labels, status, workspace, off_dev = _rocm.cluster_ragged_async(packed, offsets, mcs, ms, eps, precision,
+                                                                    flags=args.flags)
+    torch.cuda.synchronize()
+    _rocm.raise_status(int(status.item()))
+    got = to_host(labels, torch)
+    log(f"parity gate: oracle on {count} windows with {threads} threads")
+    from oracle import oracle
+
+    def one(w):
+        lab, st = oracle.hdbscan(w, mcs, ms, eps, precision=precision)
+        assert st == 0
+        return lab
+
+    with cf.ThreadPoolExecutor(threads) as pool:
+        expected = list(pool.map(one, windows))
+    exact = sum(int(np.array_equal(got[offsets[i]:offsets[i + 1]], e)) for i, e in enumerate(expected))
+    if exact != count:
+        log(f"PARITY FAILURE: {count - exact} of {count} windows differ from the oracle")
+        sys.exit(3)
+    log(f"parity gate: {exact}/{count} windows label-exact")
+    for _ in range(args.warmup):
+        _rocm.cluster_ragged_async(packed, offsets, mcs, ms, eps, precision, flags=args.flags, out=labels,
+                                   workspace=workspace, status=status, offsets_dev=off_dev)
+    status.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _rocm.cluster_ragged_async(packed, offsets, mcs, ms, eps, precision, flags=args.flags, out=labels,
+                                   workspace=workspace, status=status, offsets_dev=off_dev)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _rocm.raise_status(int(status.item()))
+    value = count * args.steps / elapsed
+    log(f"timed {args.steps} steps: {elapsed / args.steps * 1e3:.3f} ms/step, {value:.0f} windows/s")
+    line = {
+        "metric": f"windows/sec ragged {count} windows N log-uniform [64, 4096] D={d} {precision}, label-exact",
+        "value": value, "unit": "windows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if precision == "float32" else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "windows": count, "points": int(offsets[-1]), "d": d,
+                   "min_cluster_size": mcs, "min_samples": ms, "cluster_selection_epsilon": eps,
+                   "flags": args.flags},
+        "label_exact_windows": exact,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,14 +233,14 @@ def main():
         return
     spec = dict(workloads.WORKLOADS[args.workload])
     if spec["n"] is None:
-        raise SystemExit("ragged workloads are not a bench line")
+        return bench_ragged(args, spec, torch)
     batch = args.batch or spec["batch"]
     n, d, precision = spec["n"], spec["d"], spec["precision"]
     mcs, ms, eps = spec["min_cluster_size"], spec["min_samples"], spec["cluster_selection_epsilon"]
     real = np.float32 if precision == "float32" else np.float64
     gen = getattr(workloads, args.workload)
     log(f"generating {args.workload}: B={batch} N={n} D={d} {precision}")
-    x = gen(seed=0, batch=batch).astype(real)
+    x = (gen(seed=0, batch=batch) if spec["batch"] > 1 else gen(seed=0)[None]).astype(real)
     xd = to_device(x, torch)
     threads = min(16, host_cores())
 
@@ -225,7 +287,7 @@ def main():
 
     log(f"parity gate: oracle on {batch} windows with {threads} threads")
     expected = oracle_labels(x, mcs, ms, eps, precision, threads)
-    value, ms_per_step, stages, exact = measure(args.flags, "MFMA filter" if not args.flags & 64 else "plain")
+    value, ms_per_step, stages, exact = measure(args.flags, "default" if not args.flags & 64 else "plain")
     plain = None
     if not args.no_plain and not args.flags & 64:
         pv, pms, pstages, _ = measure(args.flags | 64, "plain")
@@ -311,7 +373,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "batch": batch, "n": n, "d": d,
                    "min_cluster_size": mcs, "min_samples": ms, "cluster_selection_epsilon": eps,
-                   "path": "plain exact-distance" if args.flags & 64 else "MFMA Gram filter",
+                   "path": "MFMA Gram filter" if "core_gram_kernel" in stages else "plain exact-distance",
                    "flags": args.flags},
         "label_exact_windows": exact,
         "public_api_windows_per_s": api_value,

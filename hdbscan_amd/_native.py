@@ -13,9 +13,10 @@ STATUS_OVERFLOW, STATUS_NONFINITE = 1, 2
 FLAG_GLOBAL_ARENA, FLAG_MATRIX, FLAG_NO_POINT_LDS, FLAG_MATRIX_INPUT, FLAG_SERIAL_SORT = 1, 2, 4, 8, 16
 FLAG_PREFILTER = 32
 FLAG_PLAIN = 64
+FLAG_FILTER = 128
 EXPORTED = ("hdb_version", "hdb_last_error", "hdb_profile", "hdb_stage_count", "hdb_stage_name",
             "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_cluster_export", "hdb_distances",
-            "hdb_core_distances", "hdb_cluster_matrix")
+            "hdb_core_distances", "hdb_cluster_matrix", "hdb_ragged_workspace_bytes", "hdb_cluster_ragged")
 
 _lib = None
 _P = ctypes.c_void_p
@@ -61,6 +62,11 @@ def lib():
         handle.hdb_cluster_matrix.restype = _I
         handle.hdb_cluster_matrix.argtypes = [_P, _P, _P, _I64, _I64, _I, _I64, _I, ctypes.c_double,
                                               _I, _P, _P, ctypes.c_size_t, _P, _I, _P, _P]
+        handle.hdb_ragged_workspace_bytes.restype = ctypes.c_size_t
+        handle.hdb_ragged_workspace_bytes.argtypes = [_P, _I64, _I64, _I, _I32, _I]
+        handle.hdb_cluster_ragged.restype = _I
+        handle.hdb_cluster_ragged.argtypes = [_P, _P, _P, _P, _I64, _I64, _I, _I64, _I32, ctypes.c_double,
+                                              _P, _P, ctypes.c_size_t, _P, _I]
         _lib = handle
     return _lib
 

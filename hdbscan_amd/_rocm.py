@@ -128,3 +128,73 @@ def cluster_async(x, min_cluster_size, min_samples, epsilon, precision, flags=0,
         ctypes.c_void_p(status.data_ptr()), int(flags))
     _native.check(rc)
     return out, status, workspace
+
+
+def pack_windows(windows, precision):
+    """Packs a list of [N_i, D] windows into one device tensor [sum N_i, D].
+
+    Returns (packed, offsets) with ``offsets`` a host int64 array of length
+    len(windows) + 1.
+    """
+    torch = _device_ready()
+    if len(windows) == 0:
+        raise ValueError("X requires nonempty batches/features and at least 2 samples")
+    parts = [_as_device(w, precision) for w in windows]
+    d = None
+    for p in parts:
+        if p.dim() != 2:
+            raise ValueError("X must have rank 2 per window")
+        if d is None:
+            d = int(p.shape[1])
+        elif int(p.shape[1]) != d:
+            raise ValueError("every window must have the same number of features")
+    offsets = np.zeros(len(parts) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum([int(p.shape[0]) for p in parts])
+    return torch.cat(parts, dim=0).contiguous(), offsets
+
+
+def cluster_ragged_async(packed, offsets, min_cluster_size, min_samples, epsilon, precision, flags=0,
+                         out=None, workspace=None, status=None, offsets_dev=None):
+    """Enqueues one clustering of windows of uneven size (``packed`` [total, D]
+    on the device, ``offsets`` the host window offsets) without any sync.
+    Returns (labels [total], status, workspace, offsets_dev)."""
+    torch = _torch()
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    windows = len(offsets) - 1
+    total, d = (int(s) for s in packed.shape)
+    lib = _native.lib()
+    dtype = _DTYPES[precision]
+    off_ptr = offsets.ctypes.data_as(ctypes.c_void_p)
+    ws_bytes = lib.hdb_ragged_workspace_bytes(off_ptr, windows, d, dtype, int(min_samples), int(flags))
+    _native.check(0 if ws_bytes else -1)
+    device = packed.device
+    if offsets_dev is None:
+        offsets_dev = torch.from_numpy(offsets).to(device)
+    if workspace is None or workspace.numel() < ws_bytes:
+        workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+    if status is None:
+        status = torch.zeros(1, dtype=torch.int32, device=device)
+    if out is None:
+        out = torch.empty(total, dtype=torch.int32, device=device)
+    rc = lib.hdb_cluster_ragged(
+        _stream_handle(device), ctypes.c_void_p(packed.data_ptr()), off_ptr,
+        ctypes.c_void_p(offsets_dev.data_ptr()), windows, d, dtype, int(min_cluster_size), int(min_samples),
+        float(epsilon), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(workspace.data_ptr()), ws_bytes,
+        ctypes.c_void_p(status.data_ptr()), int(flags))
+    _native.check(rc)
+    return out, status, workspace, offsets_dev
+
+
+def cluster_ragged(windows, min_cluster_size, min_samples=None, cluster_selection_epsilon=0.0,
+                   precision="float32", flags=0):
+    """Labels of windows of uneven size in one call: a list of device int32
+    label tensors, one per window ([N_i]).  Backend-level: the reference's
+    public API is dense only (reference hdbscan_gpu/_api.py:71-78)."""
+    torch = _device_ready()
+    if min_samples is None:
+        min_samples = min_cluster_size
+    packed, offsets = pack_windows(windows, precision)
+    labels, status, _, _ = cluster_ragged_async(packed, offsets, min_cluster_size, min_samples,
+                                                cluster_selection_epsilon, precision, flags)
+    raise_status(int(status.item()))
+    return [labels[offsets[i]:offsets[i + 1]] for i in range(len(offsets) - 1)]

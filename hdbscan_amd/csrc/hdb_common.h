@@ -77,4 +77,41 @@ HDB_HD TreeLayout tree_layout(int n, int real_size) {
   return L;
 }
 
+// The windows of one launch.  Dense: window w holds points w*n .. w*n+n-1.
+// Ragged: window w holds points off[w] .. off[w+1]-1 and a launch covers the
+// windows whose size lies in (lo, n]; a workgroup of another window exits
+// before any barrier.
+struct Windows {
+  const int64_t* off;  // [count + 1] point offsets, or null (dense)
+  int n;               // dense size, or the largest size of the launch
+  int lo;              // ragged: sizes <= lo belong to another launch
+  int count;           // number of windows (ragged)
+  __device__ __forceinline__ bool get(int w, int64_t& base, int& nw) const {
+    if (!off) {
+      base = (int64_t)w * n;
+      nw = n;
+      return true;
+    }
+    base = off[w];
+    nw = (int)(off[w + 1] - base);
+    return nw > lo && nw <= n;
+  }
+  // Window of point p (ragged: binary search over the offsets).
+  __device__ __forceinline__ int window_of(int64_t p) const {
+    if (!off) return (int)(p / n);
+    int a = 0, b = count;
+    while (b - a > 1) {
+      const int m = (a + b) >> 1;
+      if (off[m] <= p) a = m; else b = m;
+    }
+    return a;
+  }
+};
+
+// Global tree arena and cluster slots of ragged launches are addressed per
+// point: window w with first point base uses bytes
+// [base * kArenaPerPoint + w * kArenaPerWindow, ...), which covers
+// tree_layout(n).total <= 51 n + 1100 for every n.
+constexpr size_t kArenaPerPoint = 64, kArenaPerWindow = 4096;
+
 }  // namespace hdb

@@ -194,7 +194,7 @@ __device__ __forceinline__ int gram_src(int k, int lo_segment) {
 // indices and ncand[b][i] (> kGramCap: recompute the row exactly).
 // Grid: (ceil(n / rows), batch).  M: per-lane minima kept in pass A (1, 2).
 template <int M, int DK, int RSET>
-__global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(int n, int ms, const uint16_t* pa,
+__global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windows win, int ms, const uint16_t* pa,
                                                                           const uint16_t* pb, const float* nb,
                                                                           int64_t points, uint16_t* cand,
                                                                           int32_t* ncand) {
@@ -203,13 +203,15 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(int n
   constexpr int RS = G::RS, WR = G::WR, CAP = kGramCap;
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ uint32_t sh_nmax;
-  const int b = blockIdx.y;
   const int tid = threadIdx.x, T = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
   const int row0 = blockIdx.x * W * WR;
+  int64_t wbase;
+  int n;
+  if (!win.get(blockIdx.y, wbase, n) || row0 >= n) return;
   uint16_t* bcol = reinterpret_cast<uint16_t*>(lds);          // [TJ][KS] bf16
   float* nbound = reinterpret_cast<float*>(bcol + TJ * KS);   // [2][n]
-  const size_t rbase = (size_t)b * n;
+  const size_t rbase = (size_t)wbase;
 
   if (tid == 0) sh_nmax = 0;
   // Zero padding of the column operands (never overwritten).
@@ -419,17 +421,21 @@ __device__ __forceinline__ float exact_sq(const float* xi, const float* xj, int 
 // recomputes its candidates exactly (or reads them from the exact distance
 // matrix when one is materialised) and keeps the KMAX smallest; a row
 // marked for the exact path streams every column instead.
+// Grid: (ceil(n / 256), windows).
 template <int KMAX>
-__global__ __launch_bounds__(256) void core_exact_kernel(const float* x, const float* dist, int64_t rows, int n,
-                                                         int d, int ms, const uint16_t* cand, const int32_t* ncand,
+__global__ __launch_bounds__(256) void core_exact_kernel(const float* x, const float* dist, Windows win, int d,
+                                                         int ms, const uint16_t* cand, const int32_t* ncand,
                                                          float* core, int32_t* status) {
   constexpr int CAP = kGramCap;
-  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int64_t base;
+  int n;
+  if (!win.get(blockIdx.y, base, n) || (int)(blockIdx.x * 256) >= n) return;
+  const int i = blockIdx.x * 256 + threadIdx.x;
   int bad = 0;
-  if (row < rows) {
-    const int64_t b = row / n;
-    const float* xb = x + (size_t)b * n * d;
-    const float* xi = xb + (size_t)(row - b * n) * d;
+  if (i < n) {
+    const int64_t row = base + i;
+    const float* xb = x + (size_t)base * d;
+    const float* xi = xb + (size_t)i * d;
     const float* di = dist ? dist + (size_t)row * n : nullptr;
     const int m = ncand[row];
     float lst[KMAX];

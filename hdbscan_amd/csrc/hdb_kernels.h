@@ -133,15 +133,17 @@ __device__ __forceinline__ void topk_offer(Real* lst, Real& lim, Real tot) {
 // 64-point tiles of the window through LDS.  DK > 0: dimensions padded to DK
 // (zero padding adds +0 exactly); DK == 0: any d, 32-dimension chunks.
 template <typename Real, int KMAX, int DK>
-__global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, int d, int ms, Real* core,
+__global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, Windows win, int d, int ms, Real* core,
                                                         int32_t* status) {
   constexpr int TJ = DK > 0 ? 64 : 32;
   constexpr int KC = DK > 0 ? DK : 32;
   __shared__ Real tile[TJ * KC];
-  const int b = blockIdx.y;
+  int64_t base;
+  int n;
+  if (!win.get(blockIdx.y, base, n) || (int)(blockIdx.x * blockDim.x) >= n) return;
   const int tid = threadIdx.x;
   const int i = blockIdx.x * blockDim.x + tid;
-  const Real* xb = x + (size_t)b * n * d;
+  const Real* xb = x + (size_t)base * d;
   Real lst[KMAX];
 #pragma unroll
   for (int q = 0; q < KMAX; ++q) lst[q] = Real(INFINITY);
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, int n, in
 #pragma unroll
     for (int q = 1; q < KMAX; ++q)
       if (q == ms - 1) v = lst[q];
-    core[(size_t)b * n + i] = v;
+    core[(size_t)base + i] = v;
   }
   // Every ordered pair passes through here, so this is the overflow check of
   // the whole distance matrix (Prim may skip pairs that cannot matter).
@@ -252,21 +254,63 @@ __global__ __launch_bounds__(64) void core_select_kernel(const Real* dist, int n
   if (lane == 0) core[r] = BitsOf<Real>::put(ans);
 }
 
+// Core distances for min_samples <= KMAX from a materialised matrix: one
+// wave per row.  Lane l keeps the KMAX smallest of row[l], row[l + 64], ...
+// (coalesced reads straight from memory); the ms-th smallest of the union
+// of the 64 sorted lists is then found by popping wave minima (equal heads
+// pop together, so the value at which the count first reaches ms is the
+// order statistic).  Grid: batch * n waves.
+template <typename Real, int KMAX>
+__global__ __launch_bounds__(256) void core_select_list_kernel(const Real* dist, int64_t rows, int n, int ms,
+                                                               Real* core) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  if (r >= rows) return;  // uniform per wave
+  const Real* src = dist + (size_t)r * n;
+  Real lst[KMAX];
+#pragma unroll
+  for (int q = 0; q < KMAX; ++q) lst[q] = Real(INFINITY);
+  for (int j = lane; j < n; j += 64) {
+    const Real v = src[j];
+    if (v < lst[KMAX - 1]) {
+#pragma unroll
+      for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
+      lst[0] = fmin(lst[0], v);
+    }
+  }
+  int cnt = 0;
+  Real ans = Real(INFINITY);
+  for (int it = 0; it < ms && cnt < ms; ++it) {
+    Real m = lst[0];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) m = fmin(m, __shfl_xor(m, o));
+    const bool eq = lst[0] == m;
+    cnt += __popcll(__ballot(eq));
+    ans = m;
+    if (eq) {
+#pragma unroll
+      for (int q = 0; q < KMAX - 1; ++q) lst[q] = lst[q + 1];
+      lst[KMAX - 1] = Real(INFINITY);
+    }
+  }
+  if (lane == 0) core[r] = ans;
+}
+
 struct TreeArgs {
   const int32_t* edge_u;
   const int32_t* edge_v;
   const void* edge_w;
   const int32_t* wflag;
-  int n;
+  Windows win;
   long long mcs;
   double epsilon;
   int sort_depth;
   int serial_sort;       // one-lane introsort instead of the workgroup-parallel one
   int32_t* labels;
   unsigned char* arena;  // global arena (when not in LDS)
-  size_t arena_stride;
-  unsigned char* slots;  // per-window cluster records of the condensation walks
-  size_t slot_stride;
+  size_t arena_stride;   // dense: bytes per window; 0: ragged addressing
+  unsigned char* slots;  // cluster records of the condensation walks
+  size_t slot_stride;    // dense: bytes per window; 0: per point
   int32_t* status;
   int32_t* export_ints;  // [B][10][n] or null
   void* export_reals;    // [B][4][n] or null
@@ -280,17 +324,23 @@ __global__ __launch_bounds__(256) void tree_kernel(TreeArgs a) {
   __shared__ int sh_next;
   __shared__ int sh_counters[4];
   __shared__ long long sh_clk[6];
-  const int b = blockIdx.x, tid = threadIdx.x, T = blockDim.x, n = a.n;
-  int32_t* lab = a.labels + (size_t)b * n;
+  const int b = blockIdx.x, tid = threadIdx.x, T = blockDim.x;
+  int64_t p0;
+  int n;
+  if (!a.win.get(b, p0, n)) return;
+  int32_t* lab = a.labels + (size_t)p0;
   if (a.wflag[b]) {
     for (int i = tid; i < n; i += T) lab[i] = -1;
     return;
   }
-  unsigned char* base = LDSA ? lds : a.arena + (size_t)b * a.arena_stride;
+  unsigned char* base =
+      LDSA ? lds
+           : a.arena + (a.arena_stride ? (size_t)b * a.arena_stride
+                                       : (size_t)p0 * kArenaPerPoint + (size_t)b * kArenaPerWindow);
   const TreeView<Real> t = tree_view<Real>(base, n);
-  const int32_t* eu = a.edge_u + (size_t)b * n;
-  const int32_t* ev = a.edge_v + (size_t)b * n;
-  const Real* ew = static_cast<const Real*>(a.edge_w) + (size_t)b * n;
+  const int32_t* eu = a.edge_u + (size_t)p0;
+  const int32_t* ev = a.edge_v + (size_t)p0;
+  const Real* ew = static_cast<const Real*>(a.edge_w) + (size_t)p0;
   for (int i = tid; i < n - 1; i += T) {
     EdgeKey<Real> k;
     k.w = ew[i];
@@ -356,7 +406,8 @@ __global__ __launch_bounds__(256) void tree_kernel(TreeArgs a) {
   int16_t* node_slot = t.edge_u;
   condense_prepare(t, n, a.mcs, node_slot, t.parent);
   __syncthreads();
-  ClusterSlot<Real>* slots = reinterpret_cast<ClusterSlot<Real>*>(a.slots + (size_t)b * a.slot_stride);
+  ClusterSlot<Real>* slots = reinterpret_cast<ClusterSlot<Real>*>(
+      a.slots + (a.slot_stride ? (size_t)b * a.slot_stride : (size_t)p0 * sizeof(ClusterSlot<Real>)));
   const int nl = condense_walks(t, n, slots, node_slot, t.parent, reinterpret_cast<int32_t*>(t.uf), sh_counters);
   if (tid == 0) clk[4] = wall_clock64();
   for (int i = tid; i < n - 1; i += T) {

@@ -167,39 +167,50 @@ __device__ __forceinline__ double square_bound(double b) {
 struct PrimArgs {
   const void* x;      // points [B][n][d] (points path)
   const void* dist;   // distances [B][n][n] (matrix path)
-  const void* core;   // [B][n] or null when unit_core
-  int n, d;
+  const void* core;   // per point, or null when unit_core
+  Windows win;        // window sizes and offsets (edges and core per point)
+  int d;
   int unit_core;
   int prefilter;      // certified Gram prefilter before exact distances
-  int32_t* edge_u;    // [B][n]
-  int32_t* edge_v;    // [B][n]
-  void* edge_w;       // [B][n] Real
+  int32_t* edge_u;    // per point: edge i of a window at its first point + i
+  int32_t* edge_v;
+  void* edge_w;       // Real
   int32_t* wflag;     // [B] per-window error bits
   int32_t* status;    // OR of all error bits
 };
 
 // MODE 0: points, coordinates of owned vertices in registers (DK padded dims,
-// zero padding adds +0 exactly), current vertex from LDS (XLDS) or memory.
+// zero padding adds +0 exactly).  The current vertex comes from the window's
+// points staged in LDS (XLDS) or, when they do not fit, from the wave that
+// found it: each wave's argmin lane writes its vertex's coordinates to the
+// wave's LDS slot before the step's barrier, and after the barrier every
+// thread reads the slot of the winner's wave (two slot sets by step parity:
+// a set is rewritten one step after its last read, behind a barrier).
 // MODE 1: matrix rows from device memory.
 template <typename Real, int VPT, int DK, int MODE, bool XLDS>
 __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   using C = Cand<Real>;
   const int b = blockIdx.x;
+  int64_t base;
+  int n;
+  if (!a.win.get(b, base, n)) return;
   const int tid = threadIdx.x, T = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
-  const int n = a.n, d = a.d;
+  const int d = a.d;
   const bool unit = a.unit_core != 0;
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ C red[2][16];
   __shared__ unsigned long long kmin[3];
+  constexpr bool WSLOT = MODE == 0 && !XLDS;
+  __shared__ Real wslot[WSLOT ? 2 * 16 * (DK > 0 ? DK : 1) : 1];
   __shared__ unsigned long long sh_nmax;
   __shared__ int sh_err;
   constexpr int DKS = DK > 0 ? DK : 1;
   Real* xs = reinterpret_cast<Real*>(lds);
   Real* cs = reinterpret_cast<Real*>(lds + (XLDS ? (size_t)n * DK * sizeof(Real) : 0));
-  const Real* xg = static_cast<const Real*>(a.x) + (size_t)b * n * (MODE == 0 ? d : 0);
-  const Real* dg = static_cast<const Real*>(a.dist) + (size_t)b * n * (MODE == 1 ? n : 0);
-  const Real* cg = static_cast<const Real*>(a.core) + (size_t)b * n;
+  const Real* xg = static_cast<const Real*>(a.x) + (size_t)base * (MODE == 0 ? d : 0);
+  const Real* dg = static_cast<const Real*>(a.dist) + (size_t)base * (MODE == 1 ? n : 0);
+  const Real* cg = static_cast<const Real*>(a.core) + (size_t)base;
   if (tid == 0) {
     sh_err = 0;
     sh_nmax = 0;
@@ -280,9 +291,17 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   for (int step = 0; step < n - 1; ++step) {
     Real xc[DKS];
     if (MODE == 0) {
+      if constexpr (XLDS) {
 #pragma unroll
-      for (int k = 0; k < DKS; ++k)
-        xc[k] = XLDS ? xs[cur * DK + k] : (k < d ? xg[(size_t)cur * d + k] : Real(0));
+        for (int k = 0; k < DKS; ++k) xc[k] = xs[cur * DK + k];
+      } else if (step == 0) {
+#pragma unroll
+        for (int k = 0; k < DKS; ++k) xc[k] = k < d ? xg[k] : Real(0);
+      } else {
+        const Real* sl = wslot + ((step - 1) & 1) * 16 * DKS + ((cur % T) >> 6) * DKS;
+#pragma unroll
+        for (int k = 0; k < DKS; ++k) xc[k] = sl[k];
+      }
     }
     const Real ccur = unit ? Real(0) : cs[cur];
     const Real ncur = gram ? xn[cur] : Real(0);
@@ -343,6 +362,20 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       }
     }
     c = wave_min(c);
+    if constexpr (WSLOT) {
+      // The wave's argmin vertex, if this lane owns it, publishes its
+      // coordinates (a wave with every vertex visited publishes nothing).
+      const int v = c.index();
+      if (v % T == tid) {
+        Real* sl = wslot + (step & 1) * 16 * DKS + wave * DKS;
+#pragma unroll
+        for (int s = 0; s < VPT; ++s)
+          if (v / T == s) {
+#pragma unroll
+            for (int k = 0; k < DKS; ++k) sl[k] = xr[s][k];
+          }
+      }
+    }
     C g;
     if constexpr (sizeof(Real) == 4) {
       // One 64-bit LDS atomic min per wave; slot step%3 is reset two steps
@@ -370,7 +403,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         if ((vis >> s) & 1u) bad |= 1;
         vis |= 1u << s;
         const Real gw = g.weight();
-        const size_t o = (size_t)b * n + step;
+        const size_t o = (size_t)base + step;
         a.edge_u[o] = gw < rmax ? src[s] : 0;
         a.edge_v[o] = nxt;
         static_cast<Real*>(a.edge_w)[o] = gw;

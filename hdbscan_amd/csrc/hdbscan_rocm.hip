@@ -122,6 +122,27 @@ struct Plan {
   size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, total = 0;
 };
 
+// One launch of the pipeline: the kernels' window descriptor, the grid's
+// window count, the points covered and the largest window.
+struct Launch {
+  Windows win;
+  int64_t count = 0;   // workgroups along the window axis
+  int64_t points = 0;  // points of the whole batch (per-point buffers)
+  int64_t nmax = 0;    // largest window of this launch
+};
+
+Launch dense_launch(int64_t batch, int64_t n) {
+  Launch L;
+  L.win.off = nullptr;
+  L.win.n = (int)n;
+  L.win.lo = 0;
+  L.win.count = (int)batch;
+  L.count = batch;
+  L.points = batch * n;
+  L.nmax = n;
+  return L;
+}
+
 int pad_dims(int64_t d) {
   if (d <= 8) return 8;
   if (d <= 16) return 16;
@@ -165,6 +186,9 @@ uint32_t gram_bytes(int dk, int rs, int64_t n) {
 void set_gram(Plan& P, int64_t n, int64_t d, int32_t ms, uint32_t lds_limit) {
   P.core_gram = false;
   if (!P.core || P.rs != 4 || ms < 3 || ms > 16 || d > 256 || (P.flags & HDB_FLAG_PLAIN)) return;
+  // With a materialised exact matrix the core distances are one selection
+  // away; the filter then only pays when asked for.
+  if (P.matrix && !(P.flags & HDB_FLAG_FILTER)) return;
   const int dk = d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
   const int rs = gram_rsets(dk, P.flags);
   int waves = (dk >= 64 ? 256 : 512) / 64;
@@ -194,7 +218,6 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.core_topk = P.core && ms <= 64;
   P.kmax = ms <= 8 ? 8 : ms <= 16 ? 16 : ms <= 32 ? 32 : 64;
   P.core_dk = pad_dims(d);
-  set_gram(P, n, d, ms, lds_limit);
   // Points path: owned coordinates must fit 64 VGPRs per thread.
   // Vertices per Prim thread: about 512 threads per window measured best at
   // N=1024 (fewer waves to synchronise per step, two slots of ILP); larger
@@ -215,6 +238,7 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.vpt = vpt;
   P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
   P.matrix = (flags & HDB_FLAG_MATRIX) || dk == 0 || (int64_t)vpt * dk * rs > 64 * 4;
+  set_gram(P, n, d, ms, lds_limit);
   const uint32_t core_bytes = P.core ? (uint32_t)(n * rs) : 0u;
   if (!P.matrix) {
     // points (optional) | core distances (when needed) | squared norms
@@ -261,15 +285,16 @@ int launch_distances(hipStream_t s, const Real* x, int64_t batch, int64_t n, int
 }
 
 template <typename Real, int KMAX>
-int launch_topk_k(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_t d, int ms, Real* core, int dk,
+int launch_topk_k(hipStream_t s, const Real* x, const Launch& L, int64_t d, int ms, Real* core, int dk,
                   int32_t* status) {
-  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)batch);
+  const dim3 grid((unsigned)((L.nmax + 255) / 256), (unsigned)L.count);
+  const Windows w = L.win;
   switch (dk) {
-    case 8: core_topk_kernel<Real, KMAX, 8><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
-    case 16: core_topk_kernel<Real, KMAX, 16><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
-    case 32: core_topk_kernel<Real, KMAX, 32><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
-    case 64: core_topk_kernel<Real, KMAX, 64><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
-    default: core_topk_kernel<Real, KMAX, 0><<<grid, 256, 0, s>>>(x, (int)n, (int)d, ms, core, status); break;
+    case 8: core_topk_kernel<Real, KMAX, 8><<<grid, 256, 0, s>>>(x, w, (int)d, ms, core, status); break;
+    case 16: core_topk_kernel<Real, KMAX, 16><<<grid, 256, 0, s>>>(x, w, (int)d, ms, core, status); break;
+    case 32: core_topk_kernel<Real, KMAX, 32><<<grid, 256, 0, s>>>(x, w, (int)d, ms, core, status); break;
+    case 64: core_topk_kernel<Real, KMAX, 64><<<grid, 256, 0, s>>>(x, w, (int)d, ms, core, status); break;
+    default: core_topk_kernel<Real, KMAX, 0><<<grid, 256, 0, s>>>(x, w, (int)d, ms, core, status); break;
   }
   return check_launch("core_topk_kernel");
 }
@@ -283,82 +308,116 @@ struct GramBuffers {
 };
 
 template <int M, int DK, int RSET>
-int launch_gram_k(hipStream_t s, const Plan& P, int64_t batch, int64_t n, int ms, const GramBuffers& g) {
+int launch_gram_k(hipStream_t s, const Plan& P, const Launch& L, int ms, const GramBuffers& g) {
   auto fn = core_gram_kernel<M, DK, RSET>;
   if (int r = allow_lds((const void*)fn, P.gram_lds)) return r;
   const int rows = P.gram_threads / 64 * GramShape<DK, RSET>::WR;
-  const dim3 grid((unsigned)((n + rows - 1) / rows), (unsigned)batch);
-  fn<<<grid, P.gram_threads, P.gram_lds, s>>>((int)n, ms, g.pa, g.pb, g.nb, batch * n, g.cand, g.ncand);
+  const dim3 grid((unsigned)((L.nmax + rows - 1) / rows), (unsigned)L.count);
+  fn<<<grid, P.gram_threads, P.gram_lds, s>>>(L.win, ms, g.pa, g.pb, g.nb, L.points, g.cand, g.ncand);
   return check_launch("core_gram_kernel");
 }
 
 template <int M, int DK, int RA, int RB>
-int launch_gram_r(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d, int ms,
-                  const GramBuffers& g) {
-  const int64_t points = batch * n;
-  gram_prep_kernel<DK><<<(unsigned)((points + 255) / 256), 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb);
-  if (int r = check_launch("gram_prep_kernel")) return r;
-  stage_mark(s, "gram_prep_kernel");
-  if (P.gram_rs == RB) return launch_gram_k<M, DK, RB>(s, P, batch, n, ms, g);
-  return launch_gram_k<M, DK, RA>(s, P, batch, n, ms, g);
+int launch_gram_r(hipStream_t s, const Plan& P, const Launch& L, int ms, const GramBuffers& g) {
+  if (P.gram_rs == RB) return launch_gram_k<M, DK, RB>(s, P, L, ms, g);
+  return launch_gram_k<M, DK, RA>(s, P, L, ms, g);
 }
 
 template <int M>
-int launch_gram_d(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d, int ms,
-                  const GramBuffers& g) {
+int launch_gram_d(hipStream_t s, const Plan& P, const Launch& L, int ms, const GramBuffers& g) {
   switch (P.gram_dk) {
-    case 8: return launch_gram_r<M, 8, 2, 4>(s, P, x, batch, n, d, ms, g);
-    case 16: return launch_gram_r<M, 16, 2, 4>(s, P, x, batch, n, d, ms, g);
-    case 32: return launch_gram_r<M, 32, 2, 4>(s, P, x, batch, n, d, ms, g);
-    case 64: return launch_gram_r<M, 64, 1, 2>(s, P, x, batch, n, d, ms, g);
-    case 128: return launch_gram_r<M, 128, 1, 2>(s, P, x, batch, n, d, ms, g);
-    case 256: return launch_gram_r<M, 256, 1, 1>(s, P, x, batch, n, d, ms, g);
+    case 8: return launch_gram_r<M, 8, 2, 4>(s, P, L, ms, g);
+    case 16: return launch_gram_r<M, 16, 2, 4>(s, P, L, ms, g);
+    case 32: return launch_gram_r<M, 32, 2, 4>(s, P, L, ms, g);
+    case 64: return launch_gram_r<M, 64, 1, 2>(s, P, L, ms, g);
+    case 128: return launch_gram_r<M, 128, 1, 2>(s, P, L, ms, g);
+    case 256: return launch_gram_r<M, 256, 1, 1>(s, P, L, ms, g);
   }
   return fail("internal: unsupported Gram padding");
 }
 
-// Split operands, candidates (MFMA bounds), then the exact selection.
-// Per-lane minima kept in pass A: one for min_samples <= 3, else two (the
-// row bound stays tight unless three of the nearest share a lane).
-int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist, int64_t batch, int64_t n,
-                int64_t d, int ms, float* core, unsigned char* ws, int32_t* status) {
+GramBuffers gram_buffers(const Plan& P, unsigned char* ws) {
   GramBuffers g;
   g.pa = reinterpret_cast<uint16_t*>(ws + P.off_pa);
   g.pb = reinterpret_cast<uint16_t*>(ws + P.off_pb);
   g.nb = reinterpret_cast<float*>(ws + P.off_nb);
   g.cand = reinterpret_cast<uint16_t*>(ws + P.off_cand);
   g.ncand = reinterpret_cast<int32_t*>(ws + P.off_ncand);
-  const int r = ms <= 3 ? launch_gram_d<1>(s, P, x, batch, n, d, ms, g) : launch_gram_d<2>(s, P, x, batch, n, d, ms, g);
+  return g;
+}
+
+// bf16 pieces and norm terms of every point of the batch.
+int launch_gram_prep(hipStream_t s, const Plan& P, const float* x, int64_t points, int64_t d, unsigned char* ws) {
+  const GramBuffers g = gram_buffers(P, ws);
+  const unsigned blocks = (unsigned)((points + 255) / 256);
+  switch (P.gram_dk) {
+    case 8: gram_prep_kernel<8><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    case 16: gram_prep_kernel<16><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    case 32: gram_prep_kernel<32><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    case 64: gram_prep_kernel<64><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    case 128: gram_prep_kernel<128><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    default: gram_prep_kernel<256><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+  }
+  return check_launch("gram_prep_kernel");
+}
+
+// Candidates (MFMA bounds), then the exact selection (after the prep, which
+// a ragged batch runs once for all its launches).  Per-lane minima kept in
+// pass A: one for min_samples <= 3, else two (the row bound stays tight
+// unless three of the nearest share a lane).
+int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist, const Launch& L, int64_t d,
+                int ms, float* core, unsigned char* ws, int32_t* status, bool prep) {
+  if (prep) {
+    if (int r = launch_gram_prep(s, P, x, L.points, d, ws)) return r;
+    stage_mark(s, "gram_prep_kernel");
+  }
+  const GramBuffers g = gram_buffers(P, ws);
+  const int r = ms <= 3 ? launch_gram_d<1>(s, P, L, ms, g) : launch_gram_d<2>(s, P, L, ms, g);
   if (r) return r;
   stage_mark(s, "core_gram_kernel");
-  const int64_t rows = batch * n;
-  const unsigned blocks = (unsigned)((rows + 255) / 256);
+  const dim3 grid((unsigned)((L.nmax + 255) / 256), (unsigned)L.count);
   if (ms <= 8)
-    core_exact_kernel<8><<<blocks, 256, 0, s>>>(x, dist, rows, (int)n, (int)d, ms, g.cand, g.ncand, core, status);
+    core_exact_kernel<8><<<grid, 256, 0, s>>>(x, dist, L.win, (int)d, ms, g.cand, g.ncand, core, status);
   else
-    core_exact_kernel<16><<<blocks, 256, 0, s>>>(x, dist, rows, (int)n, (int)d, ms, g.cand, g.ncand, core, status);
+    core_exact_kernel<16><<<grid, 256, 0, s>>>(x, dist, L.win, (int)d, ms, g.cand, g.ncand, core, status);
   return check_launch("core_exact_kernel");
 }
 
 template <typename Real>
-int launch_core(hipStream_t s, const Plan& P, const Real* x, int64_t batch, int64_t n, int64_t d, int ms,
-                Real* core, Real* matrix, unsigned char* ws, int32_t* status, bool matrix_ready) {
+int launch_core(hipStream_t s, const Plan& P, const Real* x, const Launch& L, int64_t d, int ms, Real* core,
+                Real* matrix, unsigned char* ws, int32_t* status, bool matrix_ready, bool gram_prep) {
   if constexpr (sizeof(Real) == 4) {
-    if (P.core_gram) return launch_gram(s, P, x, matrix_ready ? matrix : nullptr, batch, n, d, ms, core, ws, status);
+    if (P.core_gram)
+      return launch_gram(s, P, x, matrix_ready ? matrix : nullptr, L, d, ms, core, ws, status, gram_prep);
   }
   if (P.core_topk && !matrix_ready) {
     switch (P.kmax) {
-      case 8: return launch_topk_k<Real, 8>(s, x, batch, n, d, ms, core, P.core_dk, status);
-      case 16: return launch_topk_k<Real, 16>(s, x, batch, n, d, ms, core, P.core_dk, status);
-      case 32: return launch_topk_k<Real, 32>(s, x, batch, n, d, ms, core, P.core_dk, status);
-      default: return launch_topk_k<Real, 64>(s, x, batch, n, d, ms, core, P.core_dk, status);
+      case 8: return launch_topk_k<Real, 8>(s, x, L, d, ms, core, P.core_dk, status);
+      case 16: return launch_topk_k<Real, 16>(s, x, L, d, ms, core, P.core_dk, status);
+      case 32: return launch_topk_k<Real, 32>(s, x, L, d, ms, core, P.core_dk, status);
+      default: return launch_topk_k<Real, 64>(s, x, L, d, ms, core, P.core_dk, status);
     }
   }
+  // Materialised matrices exist for dense batches only.
+  if (L.win.off) return fail("internal: ragged batch on the matrix path");
+  const int64_t batch = L.count, n = L.nmax;
   if (!matrix_ready) {
     if (int r = launch_distances<Real>(s, x, batch, n, d, matrix, status)) return r;
   }
-  const uint32_t bytes = (uint32_t)(n * sizeof(Real));
-  core_select_kernel<Real><<<(unsigned)(batch * n), 64, bytes, s>>>(matrix, (int)n, ms, core);
+  const int64_t rows = batch * n;
+  const unsigned blocks = (unsigned)((rows + 3) / 4);
+  if (ms <= 8) {
+    core_select_list_kernel<Real, 8><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+  } else if (ms <= 16) {
+    core_select_list_kernel<Real, 16><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+  } else if (ms <= 32) {
+    core_select_list_kernel<Real, 32><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+  } else if (ms <= 64) {
+    core_select_list_kernel<Real, 64><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+  } else {
+    const uint32_t bytes = (uint32_t)(n * sizeof(Real));
+    core_select_kernel<Real><<<(unsigned)rows, 64, bytes, s>>>(matrix, (int)n, ms, core);
+  }
   return check_launch("core_select_kernel");
 }
 
@@ -366,7 +425,7 @@ template <typename Real, int VPT, int DK, int MODE, bool XLDS>
 int launch_prim_one(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
   auto fn = prim_kernel<Real, VPT, DK, MODE, XLDS>;
   if (int r = allow_lds((const void*)fn, P.prim_lds)) return r;
-  fn<<<(unsigned)batch, P.threads, P.prim_lds, s>>>(a);
+  fn<<<(unsigned)batch, P.threads, P.prim_lds, s>>>(a);  // batch: workgroups along the window axis
   return check_launch("prim_kernel");
 }
 
@@ -415,8 +474,9 @@ int launch_tree(hipStream_t s, const Plan& P, int64_t batch, const TreeArgs& a) 
 
 template <typename Real>
 int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_dist, const Real* given_core,
-                int64_t batch, int64_t n, int64_t d, long long mcs, int ms, bool unit, double eps,
-                int sort_depth, int32_t* labels, unsigned char* ws, int32_t* status, int32_t* exp_i, void* exp_r) {
+                const Launch& L, int64_t d, long long mcs, int ms, bool unit, double eps, int sort_depth,
+                int32_t* labels, unsigned char* ws, int32_t* status, int32_t* exp_i, void* exp_r,
+                bool gram_prep = true) {
   int32_t* eu = reinterpret_cast<int32_t*>(ws + P.off_eu);
   int32_t* ev = reinterpret_cast<int32_t*>(ws + P.off_ev);
   Real* ew = reinterpret_cast<Real*>(ws + P.off_ew);
@@ -425,18 +485,18 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   Real* matrix = reinterpret_cast<Real*>(ws + P.off_matrix);
   const Real* dist = given_dist;
   const Real* core_in = given_core;
-  g_timers.count = 0;
   stage_mark(s, "start");
   if (!given_dist) {
     bool matrix_ready = false;
     if (P.matrix) {
-      if (int r = launch_distances<Real>(s, x, batch, n, d, matrix, status)) return r;
+      if (L.win.off) return fail("internal: ragged batch on the matrix path");
+      if (int r = launch_distances<Real>(s, x, L.count, L.nmax, d, matrix, status)) return r;
       matrix_ready = true;
       dist = matrix;
       stage_mark(s, "distance_kernel");
     }
     if (!unit) {
-      if (int r = launch_core<Real>(s, P, x, batch, n, d, ms, core, matrix, ws, status, matrix_ready)) return r;
+      if (int r = launch_core<Real>(s, P, x, L, d, ms, core, matrix, ws, status, matrix_ready, gram_prep)) return r;
       core_in = core;
       stage_mark(s, P.core_gram ? "core_exact_kernel" : P.core_topk ? "core_topk_kernel" : "core_select_kernel");
     }
@@ -445,7 +505,7 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   pa.x = x;
   pa.dist = dist;
   pa.core = unit ? nullptr : core_in;
-  pa.n = (int)n;
+  pa.win = L.win;
   pa.d = (int)d;
   pa.unit_core = unit ? 1 : 0;
   pa.prefilter = (P.flags & HDB_FLAG_PREFILTER) ? 1 : 0;
@@ -454,14 +514,14 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   pa.edge_w = ew;
   pa.wflag = wflag;
   pa.status = status;
-  if (int r = launch_prim<Real>(s, P, batch, pa)) return r;
+  if (int r = launch_prim<Real>(s, P, L.count, pa)) return r;
   stage_mark(s, "prim_kernel");
   TreeArgs ta;
   ta.edge_u = eu;
   ta.edge_v = ev;
   ta.edge_w = ew;
   ta.wflag = wflag;
-  ta.n = (int)n;
+  ta.win = L.win;
   ta.mcs = mcs;
   ta.epsilon = eps;
   ta.sort_depth = sort_depth;
@@ -474,9 +534,179 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   ta.status = status;
   ta.export_ints = exp_i;
   ta.export_reals = exp_r;
-  if (int r = launch_tree<Real>(s, P, batch, ta)) return r;
+  if (int r = launch_tree<Real>(s, P, L.count, ta)) return r;
   stage_mark(s, "tree_kernel");
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Ragged batches: windows of uneven size, packed point after point.
+//
+// Windows are grouped by size into buckets (64, 128], (128, 256], ...,
+// (2048, 4096] (the first bucket also takes every size <= 64).  Each bucket
+// is one pass of the pipeline planned for its largest window; its kernels
+// run over all windows and workgroups of other buckets exit at once.  The
+// buckets are independent, so they run concurrently on internal streams
+// forked from and joined back into the caller's stream.  Per-point buffers
+// are shared by all buckets; the split bf16 operands of the Gram filter are
+// written once for the whole batch before the fork.
+
+constexpr int kBuckets = 7;
+constexpr int kBucketTop[kBuckets] = {64, 128, 256, 512, 1024, 2048, 4096};
+constexpr int kSideStreams = 4;
+
+struct RaggedPlan {
+  int64_t windows = 0, points = 0;
+  int64_t count[kBuckets] = {}, nmax[kBuckets] = {};
+  Plan plan[kBuckets];
+  bool used[kBuckets] = {};
+  bool gram = false, global_arena = false;
+  int gram_dk = 0;
+  size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0;
+  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, total = 0;
+};
+
+int bucket_of(int64_t n) {
+  int k = 0;
+  while (n > kBucketTop[k]) ++k;
+  return k;
+}
+
+int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, int32_t ms, int flags,
+                     RaggedPlan* out) {
+  if (!off) return fail("null pointer argument");
+  if (windows < 1) return fail("X requires nonempty batches/features and at least 2 samples");
+  if (off[0] != 0) return fail("window offsets must start at 0");
+  RaggedPlan R;
+  R.windows = windows;
+  for (int64_t w = 0; w < windows; ++w) {
+    const int64_t n = off[w + 1] - off[w];
+    if (n < 2) return fail("X requires nonempty batches/features and at least 2 samples");
+    if (n > HDB_MAX_SAMPLES) return fail("at most 4096 samples per window are supported");
+    if (ms > n) return fail("min_samples must be in [1, n] for every window");
+    const int k = bucket_of(n);
+    R.used[k] = true;
+    R.count[k] += 1;
+    R.nmax[k] = std::max<int64_t>(R.nmax[k], n);
+  }
+  R.points = off[windows];
+  const int rs = dtype == HDB_F32 ? 4 : 8;
+  for (int k = 0; k < kBuckets; ++k) {
+    if (!R.used[k]) continue;
+    if (make_plan(R.count[k], R.nmax[k], d, dtype, ms, flags, &R.plan[k])) return -1;
+    const Plan& P = R.plan[k];
+    if (P.matrix)
+      return fail("ragged batches need the points path: at most 16 dimensions (float32) or 8 (float64) "
+                  "for windows above 1024 points, 64 / 32 below");
+    if (P.core_gram) {
+      R.gram = true;
+      R.gram_dk = P.gram_dk;
+    }
+    if (!P.tree_lds) R.global_arena = true;
+  }
+  for (int k = 0; k < kBuckets; ++k)
+    if (R.used[k] && R.gram && !R.plan[k].core_gram) return fail("internal: mixed Gram plans in a ragged batch");
+  const size_t pts = (size_t)R.points;
+  const bool core = ms > 2;
+  size_t o = 0;
+  R.off_eu = o; o = align256(o + pts * 4);
+  R.off_ev = o; o = align256(o + pts * 4);
+  R.off_ew = o; o = align256(o + pts * rs);
+  R.off_flag = o; o = align256(o + (size_t)windows * 4);
+  R.off_core = o; o = align256(o + (core ? pts * rs : 0));
+  R.off_arena = o; o = align256(o + (R.global_arena ? pts * kArenaPerPoint + (size_t)windows * kArenaPerWindow : 0));
+  R.off_slots = o;
+  o = align256(o + pts * (rs == 4 ? sizeof(ClusterSlot<float>) : sizeof(ClusterSlot<double>)));
+  R.off_cand = o; o = align256(o + (R.gram ? pts * kGramCap * 2 : 0));
+  R.off_ncand = o; o = align256(o + (R.gram ? pts * 4 : 0));
+  const size_t pieces = R.gram ? pts * 2 * (size_t)R.gram_dk * 2 : 0;
+  R.off_pa = o; o = align256(o + pieces);
+  R.off_pb = o; o = align256(o + pieces);
+  R.off_nb = o; o = align256(o + (R.gram ? 3 * pts * 4 : 0));
+  R.total = o;
+  // Point the bucket plans at the shared buffers (per-point addressing).
+  for (int k = 0; k < kBuckets; ++k) {
+    Plan& P = R.plan[k];
+    P.off_eu = R.off_eu; P.off_ev = R.off_ev; P.off_ew = R.off_ew; P.off_flag = R.off_flag;
+    P.off_core = R.off_core; P.off_arena = R.off_arena; P.off_slots = R.off_slots;
+    P.off_cand = R.off_cand; P.off_ncand = R.off_ncand; P.off_pa = R.off_pa; P.off_pb = R.off_pb;
+    P.off_nb = R.off_nb; P.off_matrix = 0; P.total = R.total;
+    P.arena_stride = 0;
+    P.slot_stride = 0;
+  }
+  *out = R;
+  return 0;
+}
+
+// Internal streams (default priority) and fork/join events, per device.
+struct SideStreams {
+  bool ready = false;
+  hipStream_t s[kSideStreams];
+  hipEvent_t fork, join[kSideStreams];
+};
+std::mutex g_side_mutex;
+SideStreams g_side[64];
+
+int side_streams(SideStreams** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail("no current HIP device");
+  SideStreams& S = g_side[dev];
+  if (!S.ready) {
+    for (int i = 0; i < kSideStreams; ++i)
+      if (hipStreamCreateWithFlags(&S.s[i], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&S.join[i], hipEventDisableTiming) != hipSuccess)
+        return fail("cannot create internal streams");
+    if (hipEventCreateWithFlags(&S.fork, hipEventDisableTiming) != hipSuccess) return fail("cannot create events");
+    S.ready = true;
+  }
+  *out = &S;
+  return 0;
+}
+
+template <typename Real>
+int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t* off_dev, int64_t d,
+               long long mcs, int ms, double eps, int32_t* labels, unsigned char* ws, int32_t* status) {
+  std::lock_guard<std::mutex> lock(g_side_mutex);
+  SideStreams* S = nullptr;
+  if (int r = side_streams(&S)) return r;
+  const bool timers = g_timers.on;
+  g_timers.on = false;  // stage events assume one stream
+  int err = 0;
+  if constexpr (sizeof(Real) == 4) {
+    if (R.gram) {
+      int k0 = 0;
+      while (!R.used[k0]) ++k0;
+      err = launch_gram_prep(s, R.plan[k0], x, R.points, d, ws);
+    }
+  }
+  if (!err && hipEventRecord(S->fork, s) != hipSuccess) err = fail("event record failed");
+  // Largest buckets first so that they start earliest.
+  int used = 0;
+  for (int k = kBuckets - 1; k >= 0 && !err; --k) {
+    if (!R.used[k]) continue;
+    hipStream_t t = S->s[used % kSideStreams];
+    if (used < kSideStreams && hipStreamWaitEvent(t, S->fork, 0) != hipSuccess) {
+      err = fail("stream wait failed");
+      break;
+    }
+    Launch L;
+    L.win.off = off_dev;
+    L.win.n = (int)R.nmax[k];
+    L.win.lo = k == 0 ? 0 : kBucketTop[k - 1];
+    L.win.count = (int)R.windows;
+    L.count = R.windows;
+    L.points = R.points;
+    L.nmax = R.nmax[k];
+    err = run_cluster<Real>(t, R.plan[k], x, nullptr, nullptr, L, d, mcs, ms, ms <= 2, eps, -100000, labels, ws,
+                            status, nullptr, nullptr, false);
+    ++used;
+  }
+  for (int i = 0; i < std::min(used, kSideStreams); ++i) {
+    if (hipEventRecord(S->join[i], S->s[i]) != hipSuccess || hipStreamWaitEvent(s, S->join[i], 0) != hipSuccess)
+      err = err ? err : fail("stream join failed");
+  }
+  g_timers.on = timers;
+  return err;
 }
 
 }  // namespace
@@ -525,10 +755,12 @@ int hdb_cluster(void* stream, const void* x, int64_t batch, int64_t n, int64_t d
   hipStream_t s = static_cast<hipStream_t>(stream);
   unsigned char* ws = static_cast<unsigned char*>(workspace);
   const bool unit = min_samples <= 2;
+  const Launch L = dense_launch(batch, n);
+  g_timers.count = 0;
   if (dtype == HDB_F32)
-    return run_cluster<float>(s, P, static_cast<const float*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+    return run_cluster<float>(s, P, static_cast<const float*>(x), nullptr, nullptr, L, d, min_cluster_size,
                               min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
-  return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+  return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, L, d, min_cluster_size,
                              min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
 }
 
@@ -546,10 +778,12 @@ int hdb_cluster_export(void* stream, const void* x, int64_t batch, int64_t n, in
   hipStream_t s = static_cast<hipStream_t>(stream);
   unsigned char* ws = static_cast<unsigned char*>(workspace);
   const bool unit = min_samples <= 2;
+  const Launch L = dense_launch(batch, n);
+  g_timers.count = 0;
   if (dtype == HDB_F32)
-    return run_cluster<float>(s, P, static_cast<const float*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+    return run_cluster<float>(s, P, static_cast<const float*>(x), nullptr, nullptr, L, d, min_cluster_size,
                               min_samples, unit, epsilon, sort_depth, labels, ws, status, export_ints, export_reals);
-  return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, batch, n, d, min_cluster_size,
+  return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, L, d, min_cluster_size,
                              min_samples, unit, epsilon, sort_depth, labels, ws, status, export_ints, export_reals);
 }
 
@@ -580,7 +814,8 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
   // The cluster plan with a materialised matrix lays out the matrix (for
   // min_samples > 64) and the Gram candidate lists (3 <= min_samples <= 16).
   Plan P;
-  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX, &P)) return -1;
+  const int filter = (flags & HDB_FLAG_PLAIN) ? 0 : HDB_FLAG_FILTER;
+  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX | filter, &P)) return -1;
   P.core = true;
   P.core_topk = min_samples <= 64;
   P.kmax = min_samples <= 8 ? 8 : min_samples <= 16 ? 16 : min_samples <= 32 ? 32 : 64;
@@ -589,12 +824,39 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
   if (workspace_bytes < need || (need && !workspace)) return fail("workspace too small");
   hipStream_t s = static_cast<hipStream_t>(stream);
   unsigned char* ws = static_cast<unsigned char*>(workspace);
+  const Launch L = dense_launch(batch, n);
   if (dtype == HDB_F32)
-    return launch_core<float>(s, P, static_cast<const float*>(x), batch, n, d, min_samples, static_cast<float*>(core),
-                              reinterpret_cast<float*>(ws + P.off_matrix), ws, status, false);
-  return launch_core<double>(s, P, static_cast<const double*>(x), batch, n, d, min_samples,
-                             static_cast<double*>(core), reinterpret_cast<double*>(ws + P.off_matrix), ws, status,
-                             false);
+    return launch_core<float>(s, P, static_cast<const float*>(x), L, d, min_samples, static_cast<float*>(core),
+                              reinterpret_cast<float*>(ws + P.off_matrix), ws, status, false, true);
+  return launch_core<double>(s, P, static_cast<const double*>(x), L, d, min_samples, static_cast<double*>(core),
+                             reinterpret_cast<double*>(ws + P.off_matrix), ws, status, false, true);
+}
+
+size_t hdb_ragged_workspace_bytes(const int64_t* offsets, int64_t windows, int64_t d, int dtype,
+                                  int32_t min_samples, int flags) {
+  g_error.clear();
+  RaggedPlan R;
+  if (make_ragged_plan(offsets, windows, d, dtype, min_samples, flags, &R)) return 0;
+  return R.total;
+}
+
+int hdb_cluster_ragged(void* stream, const void* x, const int64_t* offsets, const int64_t* offsets_dev,
+                       int64_t windows, int64_t d, int dtype, int64_t min_cluster_size, int32_t min_samples,
+                       double epsilon, int32_t* labels, void* workspace, size_t workspace_bytes, int32_t* status,
+                       int flags) {
+  g_error.clear();
+  if (!x || !labels || !status || !offsets_dev) return fail("null pointer argument");
+  if (min_cluster_size < 2) return fail("min_cluster_size must be >= 2");
+  RaggedPlan R;
+  if (make_ragged_plan(offsets, windows, d, dtype, min_samples, flags, &R)) return -1;
+  if (workspace_bytes < R.total || !workspace) return fail("workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
+  if (dtype == HDB_F32)
+    return run_ragged<float>(s, R, static_cast<const float*>(x), offsets_dev, d, min_cluster_size, min_samples,
+                             epsilon, labels, ws, status);
+  return run_ragged<double>(s, R, static_cast<const double*>(x), offsets_dev, d, min_cluster_size, min_samples,
+                            epsilon, labels, ws, status);
 }
 
 int hdb_cluster_matrix(void* stream, const void* distances, const void* core, int64_t batch, int64_t n, int dtype,
@@ -611,12 +873,14 @@ int hdb_cluster_matrix(void* stream, const void* distances, const void* core, in
   if (workspace_bytes < P.total || !workspace) return fail("workspace too small");
   hipStream_t s = static_cast<hipStream_t>(stream);
   unsigned char* ws = static_cast<unsigned char*>(workspace);
+  const Launch L = dense_launch(batch, n);
+  g_timers.count = 0;
   if (dtype == HDB_F32)
     return run_cluster<float>(s, P, nullptr, static_cast<const float*>(distances), static_cast<const float*>(core),
-                              batch, n, 1, min_cluster_size, unit_core ? 1 : 3, unit_core != 0, epsilon, sort_depth,
+                              L, 1, min_cluster_size, unit_core ? 1 : 3, unit_core != 0, epsilon, sort_depth,
                               labels, ws, status, export_ints, export_reals);
   return run_cluster<double>(s, P, nullptr, static_cast<const double*>(distances), static_cast<const double*>(core),
-                             batch, n, 1, min_cluster_size, unit_core ? 1 : 3, unit_core != 0, epsilon, sort_depth,
+                             L, 1, min_cluster_size, unit_core ? 1 : 3, unit_core != 0, epsilon, sort_depth,
                              labels, ws, status, export_ints, export_reals);
 }
 

@@ -50,6 +50,7 @@ extern "C" {
 #define HDB_FLAG_SERIAL_SORT 16 /* one-lane introsort (the parallel one is default) */
 #define HDB_FLAG_PREFILTER 32   /* Prim: certified FMA Gram bound skips exact distances */
 #define HDB_FLAG_PLAIN 64       /* no MFMA Gram filter: every core distance exact */
+#define HDB_FLAG_FILTER 128     /* MFMA Gram filter also on the matrix path */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
  * 3 -> 4; 0 = 2; clamped to what the dimension allows). */
@@ -105,13 +106,35 @@ int hdb_distances(void *stream, const void *x, int64_t batch, int64_t n,
 
 /* Core distances core[batch][n]: the (min_samples-1)-th smallest distance of
  * each row, self included.  Workspace: hdb_workspace_bytes(batch, n, d,
- * dtype, min_samples, flags | HDB_FLAG_MATRIX) bytes always suffice (none
- * is used when min_samples <= 64 and the MFMA filter does not apply).
+ * dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_FILTER) bytes
+ * always suffice (none is used when min_samples <= 64 and the MFMA filter
+ * does not apply).  The filter is used unless flags has HDB_FLAG_PLAIN.
  * Replaces reference hdbscan_gpu/_cuda.py:216-220 (cp.partition). */
 int hdb_core_distances(void *stream, const void *x, int64_t batch, int64_t n,
                        int64_t d, int dtype, int32_t min_samples, void *core,
                        void *workspace, size_t workspace_bytes, int32_t *status,
                        int flags);
+
+/* Ragged batch: `windows` windows of uneven size packed point after point in
+ * x[total][d] (row-major), window w holding points offsets[w] ..
+ * offsets[w+1]-1 (offsets[0] = 0, each size 2..4096, min_samples <= size).
+ * `offsets` is the host copy (planning), `offsets_dev` the same values in
+ * device memory (read by the kernels).  Labels are written per point into
+ * labels[total], numbered within each window as hdb_cluster numbers them.
+ * Windows are grouped by size; the groups run concurrently on internal
+ * streams forked from and joined into `stream`.  Windows above 1024 points
+ * need d <= 16 (float32) or d <= 8 (float64) (the points path).
+ * The reference API is dense only (hdbscan_gpu/_api.py:71-78: one [B, N, D]
+ * array); this entry serves uneven windows in one call. */
+size_t hdb_ragged_workspace_bytes(const int64_t *offsets, int64_t windows,
+                                  int64_t d, int dtype, int32_t min_samples,
+                                  int flags);
+int hdb_cluster_ragged(void *stream, const void *x, const int64_t *offsets,
+                       const int64_t *offsets_dev, int64_t windows, int64_t d,
+                       int dtype, int64_t min_cluster_size,
+                       int32_t min_samples, double epsilon, int32_t *labels,
+                       void *workspace, size_t workspace_bytes,
+                       int32_t *status, int flags);
 
 /* Graph and tree stages from a given distance matrix [batch][n][n] and core
  * distances [batch][n] (core may be NULL when unit_core != 0, meaning every

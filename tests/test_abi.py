@@ -42,8 +42,12 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_workspace_sizing():
     lib = _native.lib()
     assert lib.hdb_version() >= 100
-    ws = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, 0)
+    ws = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, _native.FLAG_PLAIN)
     assert ws > 256 * 1024 * 12
+    # The MFMA filter adds per point: bf16 pieces (2 x 2 x 16 x 2 B), norm
+    # terms (12 B), candidate lists (32 x 2 B) and counts (4 B).
+    wf = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, 0)
+    assert wf >= ws + 256 * 1024 * (128 + 12 + 64 + 4)
     # Materialised matrix path needs B*N*N reals more.
     wm = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, _native.FLAG_MATRIX)
     assert wm >= ws + 256 * 1024 * 1024 * 4

@@ -310,3 +310,19 @@ def test_workload_edges_bitwise(index):
     np.testing.assert_array_equal(ex[0]["edge_v"], v)
     np.testing.assert_array_equal(ex[0]["edge_w"], w)
     np.testing.assert_array_equal(labels[0], t["labels"])
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("ms", [3, 5, 9, 17, 40, 70])
+@pytest.mark.parametrize("flags", [2 | 64, 2, 2 | 128])
+def test_matrix_path_core_selection_edges(precision, ms, flags):
+    """Core distances selected from the materialised matrix (register lists
+    up to min_samples 64, bitwise search beyond), or the Gram filter with
+    matrix lookups: Prim edge weights (which contain them) bitwise."""
+    for name, x in prefilter_windows(precision).items():
+        labels, status, ex = G.cluster_export(x, 5, ms, 0.0, precision, flags=flags)
+        assert status == 0, name
+        (u, v, w), t = oracle_window(x, 5, ms, 0.0, precision)
+        np.testing.assert_array_equal(ex[0]["edge_w"].view(BITS[precision]), w.view(BITS[precision]), err_msg=name)
+        np.testing.assert_array_equal(ex[0]["edge_u"], u, err_msg=name)
+        np.testing.assert_array_equal(labels[0], t["labels"], err_msg=name)
