@@ -1,7 +1,7 @@
 """ROCm backend: the MI355X replacement of the reference's CUDA backend.
 
 ``cluster`` keeps the reference backend contract
-(reference hdbscan_gpu/_cuda.py:193-230): it takes an already validated host
+(reference hdbscan_gpu/_cuda.py:91-128): it takes an already validated host
 array or a native device array (here a ROCm ``torch.Tensor``), returns a newly
 allocated device-resident ``int32`` label tensor ``[B, N]`` (``[N]`` for 2-D
 input), enqueues all work on the caller's current stream and raises the

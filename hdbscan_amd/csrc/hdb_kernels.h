@@ -128,7 +128,7 @@ __device__ __forceinline__ void topk_offer(Real* lst, Real& lim, Real tot) {
 
 // Core distance of row i: the (ms-1)-th smallest of d(i, j) over all j
 // (self included), exactly as an order statistic of the reference's row
-// (reference hdbscan_gpu/_cuda.py:216-220).  One thread per row keeps the
+// (reference hdbscan_gpu/_cuda.py:114-118).  One thread per row keeps the
 // KMAX smallest distances in a register insertion network while streaming
 // 64-point tiles of the window through LDS.  DK > 0: dimensions padded to DK
 // (zero padding adds +0 exactly); DK == 0: any d, 32-dimension chunks.

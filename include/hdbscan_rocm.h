@@ -19,10 +19,10 @@
  *
  * The functions replace the reference's CUDA backend entry
  *   cluster(x, min_cluster_size, min_samples, epsilon, precision)
- *   (reference hdbscan_gpu/_cuda.py:193-230, called from
+ *   (reference hdbscan_gpu/_cuda.py:91-128, called from
  *    hdbscan_gpu/_api.py:92-102)
  * and its kernel launches K1/K2 and the library k-selection
- *   (reference hdbscan_gpu/_cuda.py:208-229).
+ *   (reference hdbscan_gpu/_cuda.py:110-127).
  */
 #ifndef HDBSCAN_ROCM_H
 #define HDBSCAN_ROCM_H
@@ -62,7 +62,7 @@ int hdb_version(void);
 const char *hdb_last_error(void);
 
 /* Stage timers of the calling thread (no counterpart in the reference; the
- * reference times stages from Python, scripts/profile_cuda.py:49-126).
+ * reference times stages from Python, scripts/profile_cuda.py:17-146).
  * With hdb_profile(1), every hdb_cluster / hdb_cluster_matrix call records a
  * HIP event after each kernel stage on its stream.  After synchronising,
  * hdb_stage_ms(i) is the duration of stage i of the last call on this thread
@@ -74,14 +74,14 @@ float hdb_stage_ms(int i);
 
 /* Bytes of device workspace hdb_cluster needs for these arguments.
  * Replaces the reference's global-workspace sizing
- * (reference hdbscan_gpu/_cuda.py:164-165, 221-223). */
+ * (reference hdbscan_gpu/_cuda.py:62-64, 119-121). */
 size_t hdb_workspace_bytes(int64_t batch, int64_t n, int64_t d, int dtype,
                            int32_t min_samples, int flags);
 
 /* Labels-only Euclidean EOM HDBSCAN of `batch` independent windows.
  * labels: [batch][n] int32, -1 = noise.  min_samples is self-inclusive;
  * epsilon is converted to the working precision.
- * Replaces reference hdbscan_gpu/_cuda.py:193-230 (cluster). */
+ * Replaces reference hdbscan_gpu/_cuda.py:91-128 (cluster). */
 int hdb_cluster(void *stream, const void *x, int64_t batch, int64_t n, int64_t d,
                 int dtype, int64_t min_cluster_size, int32_t min_samples,
                 double epsilon, int32_t *labels, void *workspace,
@@ -90,7 +90,7 @@ int hdb_cluster(void *stream, const void *x, int64_t batch, int64_t n, int64_t d
 /* hdb_cluster with every intermediate exported (layout of hdb_cluster_matrix
  * below) and an optional forced introsort depth: the white-box view of the
  * production path.  Replaces the reference's diagnostic kernels that export
- * the cluster kernel's workspace (reference tests/test_kernels.py:197-266). */
+ * the cluster kernel's workspace (reference tests/test_kernels.py:197-270). */
 int hdb_cluster_export(void *stream, const void *x, int64_t batch, int64_t n,
                        int64_t d, int dtype, int64_t min_cluster_size,
                        int32_t min_samples, double epsilon, int sort_depth,
@@ -109,7 +109,7 @@ int hdb_distances(void *stream, const void *x, int64_t batch, int64_t n,
  * dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_FILTER) bytes
  * always suffice (none is used when min_samples <= 64 and the MFMA filter
  * does not apply).  The filter is used unless flags has HDB_FLAG_PLAIN.
- * Replaces reference hdbscan_gpu/_cuda.py:216-220 (cp.partition). */
+ * Replaces reference hdbscan_gpu/_cuda.py:114-118 (cp.partition). */
 int hdb_core_distances(void *stream, const void *x, int64_t batch, int64_t n,
                        int64_t d, int dtype, int32_t min_samples, void *core,
                        void *workspace, size_t workspace_bytes, int32_t *status,
@@ -145,8 +145,8 @@ int hdb_cluster_ragged(void *stream, const void *x, const int64_t *offsets,
  *   export_reals [batch][4][n] Real:  edge_w, distance, birth, stability
  * (cluster-indexed arrays hold cluster id - n).  sort_depth < -1000 keeps
  * NumPy's depth; other values force the introsort depth.
- * Replaces reference hdbscan_gpu/_cuda.py:224-229 (clusters_kernel launch)
- * and the diagnostic workspace export of reference tests/test_kernels.py:197-266. */
+ * Replaces reference hdbscan_gpu/_cuda.py:122-127 (clusters_kernel launch)
+ * and the diagnostic workspace export of reference tests/test_kernels.py:197-270. */
 int hdb_cluster_matrix(void *stream, const void *distances, const void *core,
                        int64_t batch, int64_t n, int dtype, int64_t min_cluster_size,
                        int unit_core, double epsilon, int sort_depth,
