@@ -376,9 +376,9 @@ __global__ __launch_bounds__(256) void tree_kernel(TreeArgs a) {
     clk[1] = wall_clock64();
     linkage_gather(t, n, su, sv, rsize);
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {
       clk[2] = wall_clock64();
-      linkage(t, n, su, sv, rsize);
+      linkage_wave(t, n, su, sv, rsize);
       clk[3] = wall_clock64();
     }
   }

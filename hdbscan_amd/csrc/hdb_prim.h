@@ -164,6 +164,14 @@ __device__ __forceinline__ double square_bound(double b) {
   return isfinite(p) ? __longlong_as_double(__double_as_longlong(p) + 1ll) : p;
 }
 
+// Vertex state moved by the periodic repacking of the points path.
+template <typename Real> struct PackRec {
+  Real best;
+  Real lim;
+  int32_t id;
+  int32_t src;
+};
+
 struct PrimArgs {
   const void* x;      // points [B][n][d] (points path)
   const void* dist;   // distances [B][n][n] (matrix path)
@@ -172,6 +180,8 @@ struct PrimArgs {
   int d;
   int unit_core;
   int prefilter;      // certified Gram prefilter before exact distances
+  int repack;         // points path: repack live vertices every `repack` steps (0: never)
+  void* pack;         // per point PackRec<Real> scratch for the repacking
   int32_t* edge_u;    // per point: edge i of a window at its first point + i
   int32_t* edge_v;
   void* edge_w;       // Real
@@ -186,6 +196,13 @@ struct PrimArgs {
 // wave's LDS slot before the step's barrier, and after the barrier every
 // thread reads the slot of the winner's wave (two slot sets by step parity:
 // a set is rewritten one step after its last read, behind a barrier).
+// Repacking (points path): every `repack` steps the unvisited vertices are
+// renumbered over the workgroup's (slot, thread) positions, vertices that
+// can still improve (best > own core) first and the others (best == own
+// core: no weight can be below it) from the top, so that waves whose
+// vertices can no longer improve skip the distance work as a whole.  Keys
+// carry the original vertex ids, so the result does not depend on where a
+// vertex sits.
 // MODE 1: matrix rows from device memory.
 template <typename Real, int VPT, int DK, int MODE, bool XLDS>
 __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
@@ -205,6 +222,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   __shared__ Real wslot[WSLOT ? 2 * 16 * (DK > 0 ? DK : 1) : 1];
   __shared__ unsigned long long sh_nmax;
   __shared__ int sh_err;
+  __shared__ int sh_pack[2][2];  // [parity] live count, count from the top
   constexpr int DKS = DK > 0 ? DK : 1;
   Real* xs = reinterpret_cast<Real*>(lds);
   Real* cs = reinterpret_cast<Real*>(lds + (XLDS ? (size_t)n * DK * sizeof(Real) : 0));
@@ -215,6 +233,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     sh_err = 0;
     sh_nmax = 0;
     kmin[0] = kmin[1] = kmin[2] = ~0ull;
+    sh_pack[0][0] = sh_pack[0][1] = sh_pack[1][0] = sh_pack[1][1] = 0;
   }
   __syncthreads();
   int err = 0;
@@ -227,16 +246,23 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     }
   }
   Real* xn = cs + (unit ? 0 : n);  // squared norms of all points (points path)
+  // Points path without LDS points: the wave holding each vertex (the wave
+  // that publishes it when it wins a step).
+  uint8_t* owner = reinterpret_cast<uint8_t*>(xn + n);
+  if (WSLOT)
+    for (int j = tid; j < n; j += T) owner[j] = (uint8_t)((j % T) >> 6);
   if (!unit)
     for (int i = tid; i < n; i += T) cs[i] = cg[i];
 
   Real xr[VPT][DKS];
   Real best[VPT], cr[VPT], lim[VPT], nr[VPT];
   int src[VPT];
+  int vid[VPT];  // vertex held by each slot (n: none)
   uint32_t vis = 0;
 #pragma unroll
   for (int s = 0; s < VPT; ++s) {
     const int j = tid + s * T;
+    vid[s] = j < n ? j : n;
     best[s] = Real(INFINITY);
     lim[s] = Real(INFINITY);
     src[s] = 1;
@@ -287,7 +313,8 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   const Real gram_k = Real(2 * (DKS + 4)) * Unit<Real>::u;
 
   const Real rmax = RealTraits<Real>::max_value();
-  int cur = 0, par = 0, bad = 0;
+  int cur = 0, par = 0, bad = 0, pp = 0;
+  PackRec<Real>* pk = static_cast<PackRec<Real>*>(a.pack) + base;
   for (int step = 0; step < n - 1; ++step) {
     Real xc[DKS];
     if (MODE == 0) {
@@ -298,9 +325,50 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
 #pragma unroll
         for (int k = 0; k < DKS; ++k) xc[k] = k < d ? xg[k] : Real(0);
       } else {
-        const Real* sl = wslot + ((step - 1) & 1) * 16 * DKS + ((cur % T) >> 6) * DKS;
+        const Real* sl = wslot + ((step - 1) & 1) * 16 * DKS + owner[cur] * DKS;
 #pragma unroll
         for (int k = 0; k < DKS; ++k) xc[k] = sl[k];
+      }
+    }
+    if (MODE == 0 && a.repack && step > 0 && step % a.repack == 0) {
+#pragma unroll
+      for (int s = 0; s < VPT; ++s) {
+        if (!((vis >> s) & 1u)) {
+          const bool can = best[s] > cr[s];
+          const int pos = can ? atomicAdd(&sh_pack[pp][0], 1) : n - 1 - atomicAdd(&sh_pack[pp][1], 1);
+          PackRec<Real> r;
+          r.best = best[s];
+          r.lim = lim[s];
+          r.id = vid[s];
+          r.src = src[s];
+          pk[pos] = r;
+        }
+      }
+      __syncthreads();
+      const int live_n = sh_pack[pp][0], top_n = sh_pack[pp][1];
+      if (tid == 0) sh_pack[pp ^ 1][0] = sh_pack[pp ^ 1][1] = 0;
+      pp ^= 1;
+      vis = 0;
+#pragma unroll
+      for (int s = 0; s < VPT; ++s) {
+        const int p = tid + s * T;
+        if (p < live_n || (p >= n - top_n && p < n)) {
+          const PackRec<Real> r = pk[p];
+          const int j = r.id;
+          vid[s] = j;
+          best[s] = r.best;
+          lim[s] = r.lim;
+          src[s] = r.src;
+          cr[s] = unit ? Real(0) : cs[j];
+#pragma unroll
+          for (int k = 0; k < DKS; ++k)
+            xr[s][k] = XLDS ? xs[j * DK + k] : (k < d ? xg[(size_t)j * d + k] : Real(0));
+          nr[s] = gram ? xn[j] : Real(0);
+          if (WSLOT) owner[j] = (uint8_t)wave;
+        } else {
+          vid[s] = n;
+          vis |= 1u << s;
+        }
       }
     }
     const Real ccur = unit ? Real(0) : cs[cur];
@@ -325,7 +393,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         need = live && !(approx - Real(2) * err >= lim[s] * Real(1.0000152587890625));
       }
       if (__ballot(need)) {
-        const int j = tid + s * T;
+        const int j = vid[s];
         if (MODE == 0) {
           Real tot = 0;
 #pragma unroll
@@ -357,7 +425,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         }
       }
       if (!((vis >> s) & 1u)) {
-        const C o = C::make(best[s], tid + s * T);
+        const C o = C::make(best[s], vid[s]);
         if (o.less(c)) c = o;
       }
     }
@@ -366,15 +434,13 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       // The wave's argmin vertex, if this lane owns it, publishes its
       // coordinates (a wave with every vertex visited publishes nothing).
       const int v = c.index();
-      if (v % T == tid) {
-        Real* sl = wslot + (step & 1) * 16 * DKS + wave * DKS;
+      Real* sl = wslot + (step & 1) * 16 * DKS + wave * DKS;
 #pragma unroll
-        for (int s = 0; s < VPT; ++s)
-          if (v / T == s) {
+      for (int s = 0; s < VPT; ++s)
+        if (vid[s] == v && !((vis >> s) & 1u)) {
 #pragma unroll
-            for (int k = 0; k < DKS; ++k) sl[k] = xr[s][k];
-          }
-      }
+          for (int k = 0; k < DKS; ++k) sl[k] = xr[s][k];
+        }
     }
     C g;
     if constexpr (sizeof(Real) == 4) {
@@ -397,7 +463,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     const int nxt = g.index();
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
-      if (nxt == tid + s * T) {
+      if (nxt == vid[s]) {
         // A visited winner means no vertex had a weight below REAL_MAX: the
         // graph is degenerate (only possible with non-finite weights).
         if ((vis >> s) & 1u) bad |= 1;

@@ -414,6 +414,73 @@ __device__ void linkage(const TreeView<Real>& t, int n, const int16_t* su, const
   }
 }
 
+// The same linkage run by one wave, 64 edges at a time.  Each lane finds
+// the roots of its edge's endpoints in the union-find state at the start of
+// the batch (with their sizes and current node ids); the unions are then
+// applied in edge order, each broadcast from its lane so that later lanes
+// replace a root that just lost by the winner and refresh the winner's size
+// and node id.  Every edge therefore sees exactly the roots, sizes and node
+// ids the sequential algorithm sees, and the unions (by size, ties to the
+// first endpoint's root) are the same.  Depth stays <= log2(n) without path
+// compression because unions are by size.
+template <typename Real>
+__device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, const int16_t* sv,
+                             int16_t* rsize) {
+  const int lane = threadIdx.x & 63;
+  int16_t* uf = t.uf;
+  for (int i0 = 0; i0 < n - 1; i0 += 64) {
+    const int i = i0 + lane;
+    const bool valid = i < n - 1;
+    int ra = 0, rc = 0, ca = 0, cc = 0, sa = 0, sc = 0;
+    if (valid) {
+      int a = su[i], c = sv[i];
+      for (int p = uf[a]; p != a; p = uf[a]) a = p;
+      for (int p = uf[c]; p != c; p = uf[c]) c = p;
+      ra = a;
+      rc = c;
+      ca = t.comp[a];
+      cc = t.comp[c];
+      sa = rsize[a];
+      sc = rsize[c];
+    }
+    int ol = 0, orr = 0, os = 0;
+    const int cnt = min(64, n - 1 - i0);
+    for (int k = 0; k < cnt; ++k) {
+      const uint32_t roots = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)ra | ((uint32_t)rc << 16)), k);
+      const uint32_t nodes = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)ca | ((uint32_t)cc << 16)), k);
+      const uint32_t sizes = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)sa | ((uint32_t)sc << 16)), k);
+      const int kra = (int)(roots & 0xFFFFu), krc = (int)(roots >> 16);
+      const int ksa = (int)(sizes & 0xFFFFu), ksc = (int)(sizes >> 16);
+      const int ns = ksa + ksc;
+      const int win = ksa < ksc ? krc : kra, los = ksa < ksc ? kra : krc;
+      const int node = n + i0 + k;
+      if (lane == k) {
+        ol = (int)(nodes & 0xFFFFu);
+        orr = (int)(nodes >> 16);
+        os = ns;
+        uf[los] = (int16_t)win;
+        rsize[win] = (int16_t)ns;
+        t.comp[win] = (int16_t)node;
+      }
+      if (ra == los) ra = win;
+      if (rc == los) rc = win;
+      if (ra == win) {
+        ca = node;
+        sa = ns;
+      }
+      if (rc == win) {
+        cc = node;
+        sc = ns;
+      }
+    }
+    if (valid) {
+      t.left[i] = (int16_t)ol;
+      t.right[i] = (int16_t)orr;
+      t.sizes[i] = (int16_t)os;
+    }
+  }
+}
+
 // Condensation, prepared in parallel.  After linkage the sort index of each
 // key is dead; for node n+i:
 //   keys[i].w   = lambda = 1/distance (+inf for distance 0)

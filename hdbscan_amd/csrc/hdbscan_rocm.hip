@@ -119,7 +119,8 @@ struct Plan {
   bool need_matrix = false;
   int flags = 0;
   size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0, off_matrix = 0;
-  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, total = 0;
+  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, off_pack = 0, total = 0;
+  int repack = 0;  // Prim repacking interval (points path)
 };
 
 // One launch of the pipeline: the kernels' window descriptor, the grid's
@@ -245,7 +246,8 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
     const uint64_t xs = (uint64_t)n * dk * rs;
     const uint64_t norms = (uint64_t)n * rs;
     P.xlds = !(flags & HDB_FLAG_NO_POINT_LDS) && xs + core_bytes + norms + 3072 <= lds_limit;
-    P.prim_lds = (uint32_t)((P.xlds ? xs : 0) + core_bytes + norms);
+    // Without LDS points, one byte per vertex records the wave holding it.
+    P.prim_lds = (uint32_t)((P.xlds ? xs : 0) + core_bytes + norms + (P.xlds ? 0 : align16((uint32_t)n)));
   } else {
     P.prim_lds = core_bytes;
   }
@@ -271,6 +273,11 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.off_pa = o; o = align256(o + pieces);
   P.off_pb = o; o = align256(o + pieces);
   P.off_nb = o; o = align256(o + (P.core_gram ? 3 * bn * 4 : 0));
+  // Prim repacking of the points path: flag bits 14..15 (0: every 64
+  // steps, 1: never, 2: every 32, 3: every 128).
+  const int rp = (flags >> 14) & 3;
+  P.repack = P.matrix ? 0 : rp == 1 ? 0 : rp == 2 ? 32 : rp == 3 ? 128 : 64;
+  P.off_pack = o; o = align256(o + (P.repack ? bn * (rs == 4 ? sizeof(PackRec<float>) : sizeof(PackRec<double>)) : 0));
   P.total = o;
   *p = P;
   return 0;
@@ -509,6 +516,8 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   pa.d = (int)d;
   pa.unit_core = unit ? 1 : 0;
   pa.prefilter = (P.flags & HDB_FLAG_PREFILTER) ? 1 : 0;
+  pa.repack = P.repack;
+  pa.pack = ws + P.off_pack;
   pa.edge_u = eu;
   pa.edge_v = ev;
   pa.edge_w = ew;
@@ -563,7 +572,7 @@ struct RaggedPlan {
   bool gram = false, global_arena = false;
   int gram_dk = 0;
   size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0;
-  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, total = 0;
+  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, off_pack = 0, total = 0;
 };
 
 int bucket_of(int64_t n) {
@@ -623,6 +632,7 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
   R.off_pa = o; o = align256(o + pieces);
   R.off_pb = o; o = align256(o + pieces);
   R.off_nb = o; o = align256(o + (R.gram ? 3 * pts * 4 : 0));
+  R.off_pack = o; o = align256(o + pts * (rs == 4 ? sizeof(PackRec<float>) : sizeof(PackRec<double>)));
   R.total = o;
   // Point the bucket plans at the shared buffers (per-point addressing).
   for (int k = 0; k < kBuckets; ++k) {
@@ -630,7 +640,7 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
     P.off_eu = R.off_eu; P.off_ev = R.off_ev; P.off_ew = R.off_ew; P.off_flag = R.off_flag;
     P.off_core = R.off_core; P.off_arena = R.off_arena; P.off_slots = R.off_slots;
     P.off_cand = R.off_cand; P.off_ncand = R.off_ncand; P.off_pa = R.off_pa; P.off_pb = R.off_pb;
-    P.off_nb = R.off_nb; P.off_matrix = 0; P.total = R.total;
+    P.off_nb = R.off_nb; P.off_matrix = 0; P.total = R.total; P.off_pack = R.off_pack;
     P.arena_stride = 0;
     P.slot_stride = 0;
   }

@@ -53,7 +53,9 @@ extern "C" {
 #define HDB_FLAG_FILTER 128     /* MFMA Gram filter also on the matrix path */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
- * 3 -> 4; 0 = 2; clamped to what the dimension allows). */
+ * 3 -> 4; 0 = 2; clamped to what the dimension allows).
+ * Bits 14..15: Prim repacking interval of the points path (0 -> 64 steps,
+ * 1 -> never, 2 -> 32, 3 -> 128). */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int hdb_version(void);
