@@ -302,6 +302,14 @@ def main():
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": dominant,
                     "algorithmic": f"{gram_flops:.4g} FLOP per launch (Gram 2*D*N^2 per window)"}
+    elif dominant == "prim_kernel" and "distance_kernel" in stages:
+        # Matrix path: Prim reads each unordered weight once (one row per step).
+        itemsize = 4 if precision == "float32" else 8
+        algo_bytes = batch * n * (n - 1) / 2 * itemsize
+        achieved = algo_bytes / (stages[dominant] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                    "algorithmic": f"{algo_bytes:.4g} B per launch (N(N-1)/2 weights per window)"}
     elif dominant in ("prim_kernel", "core_topk_kernel", "distance_kernel"):
         peak = FP32_PEAK_TFLOPS if precision == "float32" else FP64_PEAK_TFLOPS
         achieved = dist_flops / (stages[dominant] * 1e-3) / 1e12
