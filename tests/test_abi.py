@@ -48,9 +48,11 @@ def test_version_and_workspace_sizing():
     # terms (12 B), candidate lists (32 x 2 B) and counts (4 B).
     wf = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, 0)
     assert wf >= ws + 256 * 1024 * (128 + 12 + 64 + 4)
-    # Materialised matrix path needs B*N*N reals more.
+    # The points path keeps 16 B per point for Prim's repacking.
+    assert ws >= 256 * 1024 * (12 + 16)
+    # The materialised matrix path needs B*N*N reals (and no repacking).
     wm = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, _native.FLAG_MATRIX)
-    assert wm >= ws + 256 * 1024 * 1024 * 4
+    assert wm >= 256 * 1024 * 1024 * 4 + 256 * 1024 * 12
     assert lib.hdb_workspace_bytes(4, 1024, 16, _native.F32, 100, 0) >= 4 * 1024 * 1024 * 4
 
 
