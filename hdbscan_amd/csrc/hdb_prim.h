@@ -132,6 +132,13 @@ __device__ __forceinline__ Cand<Real> wave_min(Cand<Real> c) {
   return m;
 }
 
+// Minimum over lanes 0..15, returned wave-uniform (the other rows are not
+// read).  All 64 lanes must be active.
+template <typename Real>
+__device__ __forceinline__ Cand<Real> row0_min(Cand<Real> c) {
+  return lane_cand(row_min(c), 0);
+}
+
 // Unit roundoff of Real.
 template <typename Real> struct Unit;
 template <> struct Unit<float> { static constexpr float u = 5.9604644775390625e-08f; };  // 2^-24
@@ -476,6 +483,215 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       }
     }
     cur = nxt;
+  }
+  if (bad) atomicOr(&sh_err, 1);
+  __syncthreads();
+  if (tid == 0) {
+    a.wflag[b] = sh_err;
+    if (sh_err) atomicOr(a.status, sh_err);
+  }
+}
+
+template <typename Real>
+__device__ __forceinline__ bool cand_eq(const Cand<Real>& x, const Cand<Real>& y) {
+  return !x.less(y) && !y.less(x);
+}
+
+// c ? x : y, field by field (a select of whole structs goes through
+// scratch memory for the padded fp64 candidate).
+__device__ __forceinline__ Cand<float> cand_pick(bool c, const Cand<float>& x, const Cand<float>& y) {
+  Cand<float> r;
+  r.k = c ? x.k : y.k;
+  return r;
+}
+__device__ __forceinline__ Cand<double> cand_pick(bool c, const Cand<double>& x, const Cand<double>& y) {
+  Cand<double> r;
+  r.w = c ? x.w : y.w;
+  r.j = c ? x.j : y.j;
+  return r;
+}
+
+// Two smallest candidates seen so far (first < second).  Value selects only:
+// a branchy swap becomes a store through a selected address, which the
+// compiler places in scratch memory.
+template <typename Real>
+__device__ __forceinline__ void cand_top2(Cand<Real>& first, Cand<Real>& second, const Cand<Real>& x) {
+  const bool lt1 = x.less(first);
+  const bool lt2 = x.less(second);
+  second = cand_pick(lt1, first, cand_pick(lt2, x, second));
+  first = cand_pick(lt1, x, first);
+}
+
+// Speculative Prim over matrix rows: up to two vertices per round.
+//
+// A step of the matrix path waits on one row read from device memory.  The
+// vertex that wins the step after the current one is usually the runner-up
+// of the previous step (the current vertex rarely lowers a key below it), so
+// each round reads the row of the current vertex c AND the row of the
+// predicted next vertex p together:
+//   K'  = keys after c's row (the true keys of this step),
+//   K'' = K' after p's row, over unvisited vertices other than p.
+// One barrier exchanges per wave min(K'), and the two smallest of K''.  If
+// the exact argmin of K' is p, the prediction held: p is added with its K'
+// key, K'' are then the true keys, and argmin K'' is added as well -- the
+// same two steps, edges and tie-breaks as one vertex per step.  Otherwise the
+// argmin of K' is added and K' kept.  The second smallest key of the round's
+// final keys predicts the next round (after a miss the next round reads one
+// row and yields the prediction).
+template <typename Real, int VPT>
+__global__ __launch_bounds__(1024) void prim_spec_kernel(PrimArgs a) {
+  using C = Cand<Real>;
+  const int b = blockIdx.x;
+  int64_t base;
+  int n;
+  if (!a.win.get(b, base, n)) return;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
+  const bool unit = a.unit_core != 0;
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ C red[2][3][16];  // [round parity][min K', min K'' (or K'), second][wave]
+  __shared__ int sh_err;
+  Real* cs = reinterpret_cast<Real*>(lds);
+  const Real* dg = static_cast<const Real*>(a.dist) + (size_t)base * n;
+  const Real* cg = static_cast<const Real*>(a.core) + (size_t)base;
+  if (tid == 0) sh_err = 0;
+  if (!unit)
+    for (int i = tid; i < n; i += T) cs[i] = cg[i];
+  Real best[VPT], cr[VPT];
+  int src[VPT];
+  uint32_t vis = 0;
+#pragma unroll
+  for (int s = 0; s < VPT; ++s) {
+    const int j = tid + s * T;
+    best[s] = Real(INFINITY);
+    src[s] = 1;
+    cr[s] = j < n && !unit ? cg[j] : Real(0);
+    if (j >= n) vis |= 1u << s;
+  }
+  if (tid == 0) vis |= 1u;  // vertex 0 starts the tree
+  __syncthreads();
+
+  const Real rmax = RealTraits<Real>::max_value();
+  const C none = C::make(rmax, 0);
+  int cur = 0, pr = -1, par = 0, bad = 0;
+  for (int step = 0; step < n - 1;) {
+    // Speculate only with two or more unvisited vertices.
+    const bool spec = pr >= 0 && n - 1 - step >= 2;
+    const Real ccur = unit ? Real(0) : cs[cur];
+    const Real cpr = spec && !unit ? cs[pr] : Real(0);
+    Real d1[VPT], d2[VPT];
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      const bool un = !((vis >> s) & 1u);
+      // Rows are read only where a strictly smaller weight is possible; for
+      // p's row the test uses the keys before c's row (a superset).
+      const bool l1 = un && (unit || fmax(ccur, cr[s]) < best[s]);
+      const bool l2 = spec && un && j != pr && (unit || fmax(cpr, cr[s]) < best[s]);
+      d1[s] = l1 ? dg[(size_t)cur * n + j] : Real(0);
+      d2[s] = l2 ? dg[(size_t)pr * n + j] : Real(0);
+    }
+    Real b1[VPT], b2[VPT];
+    int s1[VPT], s2[VPT];
+    C k1a = none, k1b = none, k2a = none, k2b = none;
+    int bad2 = 0;
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      b1[s] = best[s];
+      s1[s] = src[s];
+      b2[s] = best[s];
+      s2[s] = src[s];
+      if (!((vis >> s) & 1u)) {
+        if (unit || fmax(ccur, cr[s]) < best[s]) {
+          bad |= !isfinite(d1[s]);
+          const Real w = unit ? d1[s] : fmax(fmax(ccur, cr[s]), d1[s]);
+          if (w < b1[s]) {
+            b1[s] = w;
+            s1[s] = cur;
+          }
+        }
+        // K': the two smallest predict the next round of a plain round; a
+        // speculative round needs only the smallest.
+        const C o1 = C::make(b1[s], j);
+        if (spec)
+          k1a = cand_pick(o1.less(k1a), o1, k1a);
+        else
+          cand_top2(k1a, k1b, o1);
+        if (spec && j != pr) {
+          b2[s] = b1[s];
+          s2[s] = s1[s];
+          if (unit || fmax(cpr, cr[s]) < b1[s]) {
+            bad2 |= !isfinite(d2[s]);
+            const Real w = unit ? d2[s] : fmax(fmax(cpr, cr[s]), d2[s]);
+            if (w < b2[s]) {
+              b2[s] = w;
+              s2[s] = pr;
+            }
+          }
+          cand_top2(k2a, k2b, C::make(b2[s], j));
+        }
+      }
+    }
+    // Per wave: the list that predicts (K'' when speculating, else K') to
+    // its two smallest, and min K'.
+    const C pa = cand_pick(spec, k2a, k1a), pb = cand_pick(spec, k2b, k1b);
+    const C w1 = wave_min(pa);
+    const C wc = spec ? wave_min(k1a) : w1;
+    const C w2 = wave_min(cand_pick(cand_eq(pa, w1), pb, pa));
+    if (lane == 0) {
+      red[par][0][wave] = wc;
+      red[par][1][wave] = w1;
+      red[par][2][wave] = w2;
+    }
+    __syncthreads();
+    // Across waves (a set is rewritten two rounds later, behind a barrier).
+    C qc = none, q1 = none, q2 = none;
+    if (lane < W) {
+      qc = red[par][0][lane];
+      q1 = red[par][1][lane];
+      q2 = red[par][2][lane];
+    }
+    par ^= 1;
+    const C g1 = row0_min(q1);
+    const C gc = spec ? row0_min(qc) : g1;
+    const C g2 = row0_min(cand_pick(cand_eq(q1, g1), q2, q1));
+    const bool hit = spec && gc.index() == pr;
+    // First vertex of the round: p on a hit, else argmin K'.
+    const C first = cand_pick(spec, gc, g1);
+    const int v1 = first.index();
+    const int v2 = hit ? g1.index() : -1;
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      if (j == v1) {
+        // A visited winner means no vertex had a weight below REAL_MAX: the
+        // graph is degenerate (only possible with non-finite weights).
+        if ((vis >> s) & 1u) bad |= 1;
+        vis |= 1u << s;
+        const Real gw = first.weight();
+        const size_t o = (size_t)base + step;
+        a.edge_u[o] = gw < rmax ? s1[s] : 0;
+        a.edge_v[o] = v1;
+        static_cast<Real*>(a.edge_w)[o] = gw;
+      }
+      if (j == v2) {
+        if ((vis >> s) & 1u) bad |= 1;
+        vis |= 1u << s;
+        const Real gw = g1.weight();
+        const size_t o = (size_t)base + step + 1;
+        a.edge_u[o] = gw < rmax ? s2[s] : 0;
+        a.edge_v[o] = v2;
+        static_cast<Real*>(a.edge_w)[o] = gw;
+      }
+      best[s] = hit ? b2[s] : b1[s];
+      src[s] = hit ? s2[s] : s1[s];
+    }
+    if (hit) bad |= bad2;
+    step += hit ? 2 : 1;
+    cur = hit ? v2 : v1;
+    // The next round's prediction: the runner-up of the keys just kept.
+    pr = !spec && g2.less(none) ? g2.index() : hit && g2.less(none) ? g2.index() : -1;
   }
   if (bad) atomicOr(&sh_err, 1);
   __syncthreads();

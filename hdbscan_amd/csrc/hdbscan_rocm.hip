@@ -227,9 +227,9 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   const int dk = pad_dims(d);
   P.dk = dk;
   const int want_vpt = (flags >> 8) & 15;
-  if (want_vpt != 0 && want_vpt != 1 && want_vpt != 2 && want_vpt != 4)
-    return fail("vertices per thread must be 1, 2 or 4");
-  int vpt = want_vpt;
+  if (want_vpt != 0 && want_vpt != 1 && want_vpt != 2 && want_vpt != 4 && want_vpt != 8)
+    return fail("vertices per thread must be 1, 2, 4 or 8");
+  int vpt = want_vpt == 8 ? 0 : want_vpt;
   if (!vpt) {
     vpt = n <= 512 ? 1 : n <= 1024 ? 2 : 4;
     while (vpt > 1 && (int64_t)vpt * dk * rs > 64 * 4 && (int64_t)(vpt / 2) * 1024 >= n) vpt /= 2;
@@ -239,6 +239,15 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.vpt = vpt;
   P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
   P.matrix = (flags & HDB_FLAG_MATRIX) || dk == 0 || (int64_t)vpt * dk * rs > 64 * 4;
+  // The speculative matrix-row Prim repeats its reductions in every wave:
+  // 8 vertices per thread halves the waves of a window (no coordinates are
+  // held in registers on this path).
+  if (want_vpt == 8) {
+    if (!P.matrix || (flags & HDB_FLAG_ONE_STEP))
+      return fail("8 vertices per thread need the speculative matrix path");
+    P.vpt = vpt = 8;
+    P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
+  }
   set_gram(P, n, d, ms, lds_limit);
   const uint32_t core_bytes = P.core ? (uint32_t)(n * rs) : 0u;
   if (!P.matrix) {
@@ -436,6 +445,14 @@ int launch_prim_one(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs&
   return check_launch("prim_kernel");
 }
 
+template <typename Real, int VPT>
+int launch_prim_spec(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
+  auto fn = prim_spec_kernel<Real, VPT>;
+  if (int r = allow_lds((const void*)fn, P.prim_lds)) return r;
+  fn<<<(unsigned)batch, P.threads, P.prim_lds, s>>>(a);  // batch: workgroups along the window axis
+  return check_launch("prim_spec_kernel");
+}
+
 template <typename Real, int VPT, int DK>
 int launch_prim_x(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
   if constexpr (VPT * DK * sizeof(Real) <= 64 * 4) {
@@ -447,7 +464,10 @@ int launch_prim_x(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a
 
 template <typename Real, int VPT>
 int launch_prim_v(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
-  if (P.matrix) return launch_prim_one<Real, VPT, 0, 1, false>(s, P, batch, a);
+  if (P.matrix) {
+    if (P.flags & HDB_FLAG_ONE_STEP) return launch_prim_one<Real, VPT, 0, 1, false>(s, P, batch, a);
+    return launch_prim_spec<Real, VPT>(s, P, batch, a);
+  }
   switch (P.dk) {
     case 8: return launch_prim_x<Real, VPT, 8>(s, P, batch, a);
     case 16: return launch_prim_x<Real, VPT, 16>(s, P, batch, a);
@@ -463,6 +483,7 @@ int launch_prim(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) 
     case 1: return launch_prim_v<Real, 1>(s, P, batch, a);
     case 2: return launch_prim_v<Real, 2>(s, P, batch, a);
     case 4: return launch_prim_v<Real, 4>(s, P, batch, a);
+    case 8: return launch_prim_spec<Real, 8>(s, P, batch, a);
   }
   return fail("internal: unsupported vertices per thread");
 }

@@ -51,7 +51,9 @@ extern "C" {
 #define HDB_FLAG_PREFILTER 32   /* Prim: certified FMA Gram bound skips exact distances */
 #define HDB_FLAG_PLAIN 64       /* no MFMA Gram filter: every core distance exact */
 #define HDB_FLAG_FILTER 128     /* MFMA Gram filter also on the matrix path */
-/* Bits 8..11: vertices per Prim thread (1, 2 or 4); 0 = automatic.
+#define HDB_FLAG_ONE_STEP 65536 /* matrix-path Prim: one vertex per step, no row speculation */
+/* Bits 8..11: vertices per Prim thread (1, 2 or 4; 8 on the speculative
+ * matrix path only); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
  * 3 -> 4; 0 = 2; clamped to what the dimension allows).
  * Bits 14..15: Prim repacking interval of the points path (0 -> 64 steps,

@@ -54,6 +54,8 @@ def test_version_and_workspace_sizing():
     wm = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, _native.FLAG_MATRIX)
     assert wm >= 256 * 1024 * 1024 * 4 + 256 * 1024 * 12
     assert lib.hdb_workspace_bytes(4, 1024, 16, _native.F32, 100, 0) >= 4 * 1024 * 1024 * 4
+    # 8 vertices per thread: accepted on the speculative matrix path.
+    assert lib.hdb_workspace_bytes(4, 4096, 16, _native.F64, 5, _native.FLAG_MATRIX | 8 << 8) > 0
 
 
 @pytest.mark.parametrize("args,match", [
@@ -63,6 +65,8 @@ def test_version_and_workspace_sizing():
     ((1, 8, 1, 7, 1, 0), "dtype"),
     ((1, 8, 1, 0, 9, 0), "min_samples"),
     ((1, 8, 1, 0, 2, 3 << 8), "vertices per thread"),
+    ((1, 8, 1, 0, 2, 8 << 8), "speculative matrix path"),
+    ((1, 8, 1, 0, 2, 2 | 65536 | 8 << 8), "speculative matrix path"),
 ])
 def test_invalid_sizes_report_errors(args, match):
     lib = _native.lib()

@@ -7,6 +7,7 @@ import pytest
 import gpu_helpers as G
 from golden_io import records, same_partition, workload_windows
 from oracle import oracle
+from hdbscan_amd._native import FLAG_ONE_STEP
 
 pytestmark = pytest.mark.gpu
 
@@ -147,10 +148,17 @@ def prim_matrices(size, dtype):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("size", [2, 255, 256, 257, 513, 1500, 4096])
-@pytest.mark.parametrize("vpt", [0, 1, 2, 4])
-def test_matrix_pipeline_matches_oracle(precision, size, vpt):
+@pytest.mark.parametrize("vpt", [0, 1, 2, 4, 8])
+@pytest.mark.parametrize("one_step", [0, FLAG_ONE_STEP])
+def test_matrix_pipeline_matches_oracle(precision, size, vpt, one_step):
+    """Matrix-row Prim (speculative two-row rounds, or one vertex per step)
+    on all-equal, retained-edge and quantized matrices: every tie."""
     mats, core = prim_matrices(size, precision)
-    labels, status, ex = G.cluster_matrix(mats, core, 2, 0.0, precision, flags=vpt << 8)
+    if vpt == 8 and one_step:
+        with pytest.raises(RuntimeError, match="8 vertices per thread"):
+            G.cluster_matrix(mats, core, 2, 0.0, precision, flags=(vpt << 8) | one_step)
+        return
+    labels, status, ex = G.cluster_matrix(mats, core, 2, 0.0, precision, flags=(vpt << 8) | one_step)
     assert status == 0
     for b in range(3):
         u, v, w = oracle.prim(mats[b], core[b], precision=precision)
@@ -224,7 +232,8 @@ def test_workload_windows(index):
     assert same_partition(got, sk_labels)
 
 
-@pytest.mark.parametrize("flags", [0, 1, 2, 4, 16, 32, 2 | 1, 1 << 8, 2 << 8, 4 << 8, 32 | 4 << 8])
+@pytest.mark.parametrize("flags", [0, 1, 2, 4, 16, 32, 2 | 1, 1 << 8, 2 << 8, 4 << 8, 32 | 4 << 8,
+                                   2 | FLAG_ONE_STEP, 2 | 4 << 8, 2 | 8 << 8])
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_path_variants_agree(flags, precision):
     rng = np.random.default_rng(11)
