@@ -539,7 +539,7 @@ __device__ __forceinline__ void cand_top2(Cand<Real>& first, Cand<Real>& second,
 // final keys predicts the next round (after a miss the next round reads one
 // row and yields the prediction).
 template <typename Real, int VPT>
-__global__ __launch_bounds__(1024) void prim_spec_kernel(PrimArgs a) {
+__global__ __launch_bounds__(VPT == 8 ? 512 : 1024) void prim_spec_kernel(PrimArgs a) {
   using C = Cand<Real>;
   const int b = blockIdx.x;
   int64_t base;

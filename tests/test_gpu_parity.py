@@ -233,7 +233,9 @@ def test_workload_windows(index):
 
 
 @pytest.mark.parametrize("flags", [0, 1, 2, 4, 16, 32, 2 | 1, 1 << 8, 2 << 8, 4 << 8, 32 | 4 << 8,
-                                   2 | FLAG_ONE_STEP, 2 | 4 << 8, 2 | 8 << 8])
+                                   2 | FLAG_ONE_STEP, 2 | 4 << 8, 2 | 8 << 8,
+                                   FLAG_ONE_STEP, 1 << 14, 2 << 14, 3 << 14, FLAG_ONE_STEP | 2 << 14,
+                                   1 << 8 | 2 << 14, 4 << 8 | 3 << 14])
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_path_variants_agree(flags, precision):
     rng = np.random.default_rng(11)
@@ -290,8 +292,10 @@ def prefilter_windows(precision):
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("mcs,ms,eps", [(2, 2, 0.0), (5, 5, 0.0), (4, 3, 0.1)])
 @pytest.mark.parametrize("vpt", [0, 1, 4])
-@pytest.mark.parametrize("prefilter", [0, 32])
+@pytest.mark.parametrize("prefilter", [0, 32, FLAG_ONE_STEP])
 def test_points_path_edges_bitwise(precision, mcs, ms, eps, vpt, prefilter):
+    """prefilter 0: Prim from points; 32: with the certified Gram skip;
+    FLAG_ONE_STEP: accepted (and without effect) on the points path."""
     for name, x in prefilter_windows(precision).items():
         labels, status, ex = G.cluster_export(x, mcs, ms, eps, precision, flags=(vpt << 8) | prefilter)
         assert status == 0, name
