@@ -117,19 +117,38 @@ __device__ __forceinline__ Cand<Real> row_min(Cand<Real> c) {
   return c;
 }
 
+// DPP move that writes only the rows in ROWS (others keep their value).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_rows(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWS, 0xF, false);
+}
+constexpr int kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+template <int CTRL, int ROWS>
+__device__ __forceinline__ Cand<float> dpp_cand_rows(const Cand<float>& c) {
+  Cand<float> o;
+  o.k = ((uint64_t)dpp_rows<CTRL, ROWS>((uint32_t)(c.k >> 32)) << 32) | dpp_rows<CTRL, ROWS>((uint32_t)c.k);
+  return o;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ Cand<double> dpp_cand_rows(const Cand<double>& c) {
+  Cand<double> o;
+  o.w = ((uint64_t)dpp_rows<CTRL, ROWS>((uint32_t)(c.w >> 32)) << 32) | dpp_rows<CTRL, ROWS>((uint32_t)c.w);
+  o.j = dpp_rows<CTRL, ROWS>(c.j);
+  return o;
+}
+
 // Wave-wide minimum, returned wave-uniform (scalar registers).  All 64 lanes
-// must be active.
+// must be active.  Row minima, then row broadcasts: lane 15 into row 1 and
+// lane 47 into row 3, then lane 31 into rows 2 and 3; lane 63 holds the
+// minimum of all four rows.
 template <typename Real>
 __device__ __forceinline__ Cand<Real> wave_min(Cand<Real> c) {
   c = row_min(c);
-  Cand<Real> m = lane_cand(c, 0);
-  Cand<Real> o = lane_cand(c, 16);
-  if (o.less(m)) m = o;
-  o = lane_cand(c, 32);
-  if (o.less(m)) m = o;
-  o = lane_cand(c, 48);
-  if (o.less(m)) m = o;
-  return m;
+  Cand<Real> o = dpp_cand_rows<kDppBcast15, 0xA>(c);
+  if (o.less(c)) c = o;
+  o = dpp_cand_rows<kDppBcast31, 0xC>(c);
+  if (o.less(c)) c = o;
+  return lane_cand(c, 63);
 }
 
 // Minimum over lanes 0..15, returned wave-uniform (the other rows are not
