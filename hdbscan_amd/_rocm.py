@@ -47,7 +47,8 @@ def _as_device(x, precision):
     host = np.ascontiguousarray(np.asarray(x), dtype=np.float32 if precision == "float32" else np.float64)
     # Stage through a freshly allocated pinned buffer: the copy is then a
     # plain DMA from page-locked memory on every host.
-    pinned = torch.from_numpy(host).pin_memory()
+    # (pin_memory copies; a read-only array is copied first for from_numpy.)
+    pinned = torch.from_numpy(host if host.flags.writeable else host.copy()).pin_memory()
     out = pinned.to(device=torch.device("cuda", torch.cuda.current_device()), non_blocking=True)
     torch.cuda.current_stream().synchronize()
     return out

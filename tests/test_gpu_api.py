@@ -243,3 +243,35 @@ print('ok')
     out = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, cwd=root, timeout=300)
     assert out.returncode == 0, out.stderr[-2000:]
     assert out.stdout.strip().endswith("ok")
+
+
+def test_custom_op_eager_and_compiled(torch):
+    """torch.ops.hdbscan_amd.cluster on device tensors: eager, and inside a
+    torch.compile(fullgraph=True) graph, label-exact against the oracle."""
+    import hdbscan_amd.ops  # noqa: F401
+
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.normal(size=(3, 120, 4)), rng.normal(size=(3, 80, 4)) + 6], axis=1)
+    x = x.astype(np.float32)
+    xd = torch.from_numpy(x).to("cuda")
+    eager = torch.ops.hdbscan_amd.cluster(xd, 5, 5, 0.0, "float32")
+    single = torch.ops.hdbscan_amd.cluster(xd[1], 5, 5, 0.0, "float32")
+
+    def f(t):
+        return torch.ops.hdbscan_amd.cluster(t * 1.0, 5, 5, 0.0, "float32")
+
+    compiled = torch.compile(f, backend="aot_eager", fullgraph=True)(xd)
+    out = expect_labels(eager, (3, 200), torch)
+    np.testing.assert_array_equal(expect_labels(compiled, (3, 200), torch), out)
+    np.testing.assert_array_equal(expect_labels(single, (200,), torch), out[1])
+    for b in range(3):
+        np.testing.assert_array_equal(out[b], oracle_labels(x[b], 5, 5))
+
+
+def test_custom_op_reports_nonfinite(torch):
+    import hdbscan_amd.ops  # noqa: F401
+
+    x = torch.zeros((1, 16, 2), device="cuda")
+    x[0, 3, 1] = float("nan")
+    with pytest.raises(ValueError, match="finite"):
+        torch.ops.hdbscan_amd.cluster(x, 2, 2, 0.0, "float32")
