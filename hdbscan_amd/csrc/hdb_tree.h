@@ -531,8 +531,8 @@ template <typename Real> struct ClusterSlot {
 // a cluster's stability only while visiting that path, in path order, so a
 // lane can accumulate it alone: each smaller child adds lambda - birth once
 // per point (all its points leave at the same lambda), a split adds
-// (lambda - birth) * size for the left then the right child.  Walks run level
-// by level (children clusters start after their parent's walk).  The
+// (lambda - birth) * size for the left then the right child.  A child
+// cluster's walk can start once its parent's walk reaches the split.  The
 // reference numbers new clusters in breadth-first order of the split nodes,
 // i.e. by (depth, preorder) of those nodes: split number r gives its children
 // n+1+2r (left) and n+2+2r (right).  node_slot[i] records the slot of each
@@ -541,7 +541,7 @@ template <typename Real> struct ClusterSlot {
 template <typename Real>
 __device__ int condense_walks(const TreeView<Real>& t, int n, ClusterSlot<Real>* slots, int16_t* node_slot,
                               const int16_t* pruned, int32_t* split_keys, int* counters) {
-  // counters: [0] level begin, [1] level end, [2] slots allocated, [3] splits
+  // counters: [2] slots allocated, [3] splits
   const int tid = threadIdx.x, T = blockDim.x;
   if (tid == 0) {
     ClusterSlot<Real> r;
@@ -554,69 +554,127 @@ __device__ int condense_walks(const TreeView<Real>& t, int n, ClusterSlot<Real>*
     r.pre = 0;
     r.label = n;
     slots[0] = r;
-    counters[0] = 0;
-    counters[1] = 1;
-    counters[2] = 1;
+  }
+  // One wave walks every cluster, taking clusters from a queue as soon as
+  // their parent's walk reaches its split (a level-by-level schedule would
+  // make the critical path the sum over levels of the longest walk instead
+  // of the longest root-to-leaf sum of walks).  A lane that reaches a split
+  // finishes its cluster, continues with the left child and queues the right
+  // one; idle lanes take queued clusters in queue order.  Walks are
+  // independent, so the order does not change any result.  Queue entries
+  // (split node | right child's slot << 13, and the split node's depth |
+  // preorder << 13) live in split_keys, free until the ranking below; slots
+  // are allocated in pairs from 1, so a slot's parity tells left from right.
+  if (tid < 64) {
+    const int lane = tid;
+    int32_t* queue = split_keys;
+    int qhead = 0, qtail = 0;  // queue entries taken / written (wave-uniform)
+    int nalloc = 1;            // slots allocated (wave-uniform)
+    bool active = lane == 0;
+    int s = 0, x = 2 * n - 2, d = 0, p = 0;
+    Real b = 0, stab = 0;
+    for (;;) {
+      const uint64_t idle = __ballot(!active);
+      const int avail = qtail - qhead;
+      if (avail > 0 && idle) {
+        const int r = (int)lanes_below(idle);
+        if (!active && r < avail) {
+          const int e0 = queue[2 * (qhead + r)], e1 = queue[2 * (qhead + r) + 1];
+          const int i = e0 & 0x1fff;
+          const int a = t.left[i];
+          const int sa = a < n ? 1 : t.sizes[a - n];
+          s = e0 >> 13;
+          x = t.right[i];
+          d = (e1 & 0x1fff) + 1;
+          p = (e1 >> 13) + 2 * sa;
+          b = t.keys[i].w;
+          stab = 0;
+          active = true;
+        }
+        qhead += min(avail, (int)__popcll(idle));
+      }
+      if (!__ballot(active)) break;
+      bool split = false;
+      int si = 0, sa = 0;
+      Real lambda = 0;
+      if (active) {
+        if (x < n || x > 2 * n - 2) {  // only reachable for a malformed tree
+          slots[s].stab = stab;
+          slots[s].key = -1;
+          active = false;
+        } else {
+          const int i = x - n;
+          const EdgeKey<Real> info = t.keys[i];
+          const int count = pruned[i];
+          node_slot[i] = (int16_t)s;
+          lambda = info.w;
+          if ((info.idx >> 27) & 1) {
+            const int a = t.left[i], c = t.right[i];
+            sa = a < n ? 1 : t.sizes[a - n];
+            const int sc = c < n ? 1 : t.sizes[c - n];
+            stab += (lambda - b) * Real(sa);
+            stab += (lambda - b) * Real(sc);
+            slots[s].stab = stab;
+            slots[s].key = (d << 13) | p;
+            split = true;
+            si = i;
+          } else {
+            const Real step = lambda - b;
+            for (int q = 0; q < count; ++q) stab += step;
+            const int next = info.idx & 0x1fff;
+            if (!next) {
+              slots[s].stab = stab;
+              slots[s].key = -1;
+              active = false;
+            } else {
+              p += (info.idx >> 13) & 0x3fff;
+              ++d;
+              x = next;
+            }
+          }
+        }
+      }
+      const uint64_t sm = __ballot(split);
+      if (sm) {
+        // Two slots per split in lane order; a split past the slot capacity
+        // (n) creates no children, as in a level-by-level walk.
+        const int s1 = nalloc + 2 * (int)lanes_below(sm);
+        const bool room = split && s1 + 1 < n;
+        const uint64_t pm = __ballot(room);
+        if (room) {
+          const int a = t.left[si], c = t.right[si];
+          ClusterSlot<Real> ca, cc;
+          ca.birth = cc.birth = lambda;
+          ca.stab = cc.stab = 0;
+          ca.parent = cc.parent = s;
+          ca.key = cc.key = -1;
+          ca.depth = cc.depth = d + 1;
+          ca.label = cc.label = -1;
+          ca.root = a;
+          ca.pre = p + 1;
+          cc.root = c;
+          cc.pre = p + 2 * sa;
+          slots[s1] = ca;
+          slots[s1 + 1] = cc;
+          const int q = qtail + (int)lanes_below(pm);
+          queue[2 * q] = si | ((s1 + 1) << 13);
+          queue[2 * q + 1] = d | (p << 13);
+          s = s1;
+          x = a;
+          ++d;
+          ++p;
+          b = lambda;
+          stab = 0;
+        } else if (split) {
+          active = false;
+        }
+        nalloc += 2 * (int)__popcll(sm);
+        qtail += (int)__popcll(pm);
+      }
+    }
+    if (lane == 0) counters[2] = nalloc;
   }
   __syncthreads();
-  for (;;) {
-    const int begin = counters[0], end = counters[1];
-    if (begin >= end) break;
-    for (int s = begin + tid; s < end; s += T) {
-      int x = slots[s].root, d = slots[s].depth, p = slots[s].pre;
-      const Real b = slots[s].birth;
-      Real stab = 0;
-      int key = -1;
-      for (;;) {
-        if (x < n || x > 2 * n - 2) break;  // only reachable for a malformed tree
-        const int i = x - n;
-        const EdgeKey<Real> info = t.keys[i];
-        const int count = pruned[i];
-        node_slot[i] = (int16_t)s;
-        const Real lambda = info.w;
-        if ((info.idx >> 27) & 1) {
-          const int a = t.left[i], c = t.right[i];
-          const int sa = a < n ? 1 : t.sizes[a - n];
-          const int sc = c < n ? 1 : t.sizes[c - n];
-          stab += (lambda - b) * Real(sa);
-          stab += (lambda - b) * Real(sc);
-          const int s1 = atomicAdd(&counters[2], 2);
-          if (s1 + 1 < n) {
-            ClusterSlot<Real> ca, cc;
-            ca.birth = cc.birth = lambda;
-            ca.stab = cc.stab = 0;
-            ca.parent = cc.parent = s;
-            ca.key = cc.key = -1;
-            ca.depth = cc.depth = d + 1;
-            ca.label = cc.label = -1;
-            ca.root = a;
-            ca.pre = p + 1;
-            cc.root = c;
-            cc.pre = p + 2 * sa;
-            slots[s1] = ca;
-            slots[s1 + 1] = cc;
-          }
-          key = (d << 13) | p;
-          break;
-        }
-        const Real step = lambda - b;
-        for (int q = 0; q < count; ++q) stab += step;
-        const int next = info.idx & 0x1fff;
-        if (!next) break;
-        p += (info.idx >> 13) & 0x3fff;
-        ++d;
-        x = next;
-      }
-      slots[s].stab = stab;
-      slots[s].key = key;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      counters[0] = end;
-      counters[1] = min(counters[2], n);
-    }
-    __syncthreads();
-  }
   const int nslots = min(counters[2], n);
   // Rank the splits by (depth, preorder): stage the keys in LDS, count.
   for (int s = tid; s < nslots; s += T) split_keys[s] = slots[s].key;
