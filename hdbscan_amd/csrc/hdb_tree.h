@@ -447,7 +447,6 @@ __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, 
     const int cnt = min(64, n - 1 - i0);
     for (int k = 0; k < cnt; ++k) {
       const uint32_t roots = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)ra | ((uint32_t)rc << 16)), k);
-      const uint32_t nodes = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)ca | ((uint32_t)cc << 16)), k);
       const uint32_t sizes = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)sa | ((uint32_t)sc << 16)), k);
       const int kra = (int)(roots & 0xFFFFu), krc = (int)(roots >> 16);
       const int ksa = (int)(sizes & 0xFFFFu), ksc = (int)(sizes >> 16);
@@ -455,8 +454,9 @@ __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, 
       const int win = ksa < ksc ? krc : kra, los = ksa < ksc ? kra : krc;
       const int node = n + i0 + k;
       if (lane == k) {
-        ol = (int)(nodes & 0xFFFFu);
-        orr = (int)(nodes >> 16);
+        // This lane's own node ids are the edge's children.
+        ol = ca;
+        orr = cc;
         os = ns;
         uf[los] = (int16_t)win;
         rsize[win] = (int16_t)ns;
