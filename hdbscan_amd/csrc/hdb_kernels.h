@@ -319,7 +319,7 @@ struct TreeArgs {
 // Tree stages of one window per workgroup.  Lane 0 runs the sequential
 // stages; the workgroup stages inputs and writes labels and exports.
 template <typename Real, bool LDSA>
-__global__ __launch_bounds__(256) void tree_kernel(TreeArgs a) {
+__global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ int sh_next;
   __shared__ int sh_counters[4];

@@ -149,18 +149,54 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 }
 
 // Partition of keys[lo..hi] by one full wave; returns the pivot's final index.
+__device__ __forceinline__ float lane0_value(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+}
+__device__ __forceinline__ double lane0_value(double v) {
+  const long long b = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 0);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 0);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Field by field (a copy of the padded fp64 key as a whole goes through
+// scratch memory).
+template <typename Real>
+__device__ __forceinline__ void key_exchange(EdgeKey<Real>& x, EdgeKey<Real>& y) {
+  const Real w = x.w;
+  const auto i = x.idx;
+  x.w = y.w;
+  x.idx = y.idx;
+  y.w = w;
+  y.idx = i;
+}
+
+// Spans handed to a partition have hi - lo > 15, so lo, mid, hi - 1 and hi
+// are four distinct positions.
 template <typename Real>
 __device__ int wave_partition(EdgeKey<Real>* keys, int lo, int hi, int16_t* lpos, int16_t* rpos, int lane) {
+  // NumPy's median of three (swap mid/lo, hi/mid, mid/lo when strictly
+  // smaller, then park the median at hi - 1), on registers: the four keys
+  // are read at once and written back once.
+  Real piv = 0;
   if (lane == 0) {
     const int mid = lo + ((hi - lo) >> 1);
-    if (keys[mid].w < keys[lo].w) key_swap(keys, mid, lo);
-    if (keys[hi].w < keys[mid].w) key_swap(keys, hi, mid);
-    if (keys[mid].w < keys[lo].w) key_swap(keys, mid, lo);
-    key_swap(keys, mid, hi - 1);
+    EdgeKey<Real> a = keys[lo], m = keys[mid], h = keys[hi];
+    const Real park_w = keys[hi - 1].w;
+    const auto park_idx = keys[hi - 1].idx;
+    if (m.w < a.w) key_exchange(m, a);
+    if (h.w < m.w) key_exchange(h, m);
+    if (m.w < a.w) key_exchange(m, a);
+    keys[lo] = a;
+    keys[hi] = h;
+    keys[mid].w = park_w;
+    keys[mid].idx = park_idx;
+    keys[hi - 1] = m;
+    piv = m.w;
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  const Real pivot = keys[hi - 1].w;
+  const Real pivot = lane0_value(piv);
   // Positions >= pivot in [lo+1, hi-1], ascending.
   int n_ge = 0;
   for (int base = lo + 1; base <= hi - 1; base += 64) {

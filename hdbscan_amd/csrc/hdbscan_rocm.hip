@@ -27,7 +27,7 @@ using namespace hdb;
 
 namespace {
 
-constexpr int kTreeThreads = 256;
+constexpr int kTreeThreads = 1024;
 
 thread_local std::string g_error;
 
