@@ -334,12 +334,13 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
           for (int g = 0; g < RS; ++g)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              if constexpr (M == 1) {
-                mn[g][r][0] = fminf(mn[g][r][0], acc[g][r]);
-              } else {
-                mn[g][r][1] = fminf(mn[g][r][1], fmaxf(mn[g][r][0], acc[g][r]));
-                mn[g][r][0] = fminf(mn[g][r][0], acc[g][r]);
-              }
+              // With mn[0] <= mn[1], the second smallest of {mn[0], mn[1], v}
+              // is their median (one instruction); the smallest is a compare
+              // and select.  No NaN quieting is needed: the filter runs only
+              // on finite bounds (columns past the window add +inf).
+              const float v = acc[g][r];
+              if constexpr (M == 2) mn[g][r][1] = __builtin_amdgcn_fmed3f(mn[g][r][0], mn[g][r][1], v);
+              mn[g][r][0] = v < mn[g][r][0] ? v : mn[g][r][0];
             }
         } else {
           // Candidates of row (set g, 4 grp + r): the 16 lanes of a group
