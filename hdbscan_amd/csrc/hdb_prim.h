@@ -68,9 +68,11 @@ template <> struct Cand<double> {
 };
 
 // DPP lane moves within 16-lane rows (quad_perm xor 1, xor 2; row_ror 4, 8).
+// Every lane is written (all rows and banks, sources inside the row), so no
+// old value is needed: mov_dpp avoids copying v into the destination first.
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_move(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
 }
 constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRor4 = 0x124, kDppRor8 = 0x128;
 
@@ -117,10 +119,11 @@ __device__ __forceinline__ Cand<Real> row_min(Cand<Real> c) {
   return c;
 }
 
-// DPP move that writes only the rows in ROWS (others keep their value).
+// DPP move that writes only the rows in ROWS; the other rows hold undefined
+// values (wave_min reads only lanes that every step wrote).
 template <int CTRL, int ROWS>
 __device__ __forceinline__ uint32_t dpp_rows(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWS, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, ROWS, 0xF, false);
 }
 constexpr int kDppBcast15 = 0x142, kDppBcast31 = 0x143;
 template <int CTRL, int ROWS>
@@ -140,7 +143,9 @@ __device__ __forceinline__ Cand<double> dpp_cand_rows(const Cand<double>& c) {
 // Wave-wide minimum, returned wave-uniform (scalar registers).  All 64 lanes
 // must be active.  Row minima, then row broadcasts: lane 15 into row 1 and
 // lane 47 into row 3, then lane 31 into rows 2 and 3; lane 63 holds the
-// minimum of all four rows.
+// minimum of all four rows.  Rows 0 and 2 after the first broadcast, and
+// rows 0 and 1 after the second, are never read: row 3 takes its own row
+// minimum, row 2's (lane 47) and row 1's combined value (lane 31).
 template <typename Real>
 __device__ __forceinline__ Cand<Real> wave_min(Cand<Real> c) {
   c = row_min(c);
