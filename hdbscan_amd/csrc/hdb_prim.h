@@ -410,7 +410,10 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       // Only a weight strictly below best can change this vertex.  With core
       // distances, max(core_cur, core_j) >= best rules it out before any
       // distance is formed (w = max(core_cur, core_j, d) >= best).
-      const bool live = !((vis >> s) & 1u) && (unit || fmax(ccur, cr[s]) < best[s]);
+      // As one comparison: visited slots compare +inf, and with unit core
+      // distances -inf keeps every unvisited vertex live.
+      const Real lv = ((vis >> s) & 1u) ? Real(INFINITY) : unit ? Real(-INFINITY) : fmax(ccur, cr[s]);
+      const bool live = lv < best[s];
       bool need = live;
       if (gram && __ballot(live)) {
         // Certified skip: a - 2E >= lim (1 + 2^-16) implies the exact
@@ -435,7 +438,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
           bad |= live && !isfinite(tot);
           // tot >= lim[s] (an upper bound of best^2) implies sqrt(tot) >= best:
           // no update, so the correctly rounded square root is skipped.
-          const bool near = live && tot < lim[s];
+          const bool near = tot < (live ? lim[s] : Real(-INFINITY));
           if (__ballot(near)) {
             const Real dd = sqrt(tot);
             const Real w = unit ? dd : fmax(fmax(ccur, cr[s]), dd);
