@@ -43,6 +43,12 @@ template <> struct Cand<float> {
   }
   __device__ __forceinline__ int index() const { return (int)(uint32_t)k; }
   __device__ __forceinline__ float weight() const { return __uint_as_float((uint32_t)(k >> 32)); }
+  // Above every (weight, vertex) key, the REAL_MAX start value included.
+  __device__ __forceinline__ static Cand none() {
+    Cand c;
+    c.k = ~0ull;
+    return c;
+  }
 };
 
 template <> struct Cand<double> {
@@ -65,7 +71,27 @@ template <> struct Cand<double> {
   }
   __device__ __forceinline__ int index() const { return (int)j; }
   __device__ __forceinline__ double weight() const { return __longlong_as_double((long long)w); }
+  __device__ __forceinline__ static Cand none() {
+    Cand c;
+    c.w = ~0ull;
+    c.j = ~0u;
+    return c;
+  }
 };
+
+// c ? x : y, field by field (a select of whole structs goes through
+// scratch memory for the padded fp64 candidate).
+__device__ __forceinline__ Cand<float> cand_pick(bool c, const Cand<float>& x, const Cand<float>& y) {
+  Cand<float> r;
+  r.k = c ? x.k : y.k;
+  return r;
+}
+__device__ __forceinline__ Cand<double> cand_pick(bool c, const Cand<double>& x, const Cand<double>& y) {
+  Cand<double> r;
+  r.w = c ? x.w : y.w;
+  r.j = c ? x.j : y.j;
+  return r;
+}
 
 // DPP lane moves within 16-lane rows (quad_perm xor 1, xor 2; row_ror 4, 8).
 // Every lane is written (all rows and banks, sources inside the row), so no
@@ -234,10 +260,13 @@ struct PrimArgs {
 // vertices can no longer improve skip the distance work as a whole.  Keys
 // carry the original vertex ids, so the result does not depend on where a
 // vertex sits.
-// MODE 1: matrix rows from device memory.
+// MODE 1: matrix rows from device memory.  MODE 2: the points path with the
+// certified FMA Gram prefilter (a separate instantiation keeps the plain
+// path free of its branches).
 template <typename Real, int VPT, int DK, int MODE, bool XLDS>
 __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   using C = Cand<Real>;
+  constexpr bool PTS = MODE != 1;  // points path (MODE 0, or 2 with the Gram prefilter)
   const int b = blockIdx.x;
   int64_t base;
   int n;
@@ -249,7 +278,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   __shared__ C red[2][16];
   __shared__ unsigned long long kmin[3];
-  constexpr bool WSLOT = MODE == 0 && !XLDS;
+  constexpr bool WSLOT = PTS && !XLDS;
   __shared__ Real wslot[WSLOT ? 2 * 16 * (DK > 0 ? DK : 1) : 1];
   __shared__ unsigned long long sh_nmax;
   __shared__ int sh_err;
@@ -257,7 +286,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   constexpr int DKS = DK > 0 ? DK : 1;
   Real* xs = reinterpret_cast<Real*>(lds);
   Real* cs = reinterpret_cast<Real*>(lds + (XLDS ? (size_t)n * DK * sizeof(Real) : 0));
-  const Real* xg = static_cast<const Real*>(a.x) + (size_t)base * (MODE == 0 ? d : 0);
+  const Real* xg = static_cast<const Real*>(a.x) + (size_t)base * (PTS ? d : 0);
   const Real* dg = static_cast<const Real*>(a.dist) + (size_t)base * (MODE == 1 ? n : 0);
   const Real* cg = static_cast<const Real*>(a.core) + (size_t)base;
   if (tid == 0) {
@@ -268,7 +297,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   }
   __syncthreads();
   int err = 0;
-  if constexpr (MODE == 0 && XLDS) {
+  if constexpr (PTS && XLDS) {
     for (int idx = tid; idx < n * DKS; idx += T) {
       const int i = idx / DKS, k = idx - i * DKS;
       const Real v = k < d ? xg[(size_t)i * d + k] : Real(0);
@@ -289,6 +318,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   Real best[VPT], cr[VPT], lim[VPT], nr[VPT];
   int src[VPT];
   int vid[VPT];  // vertex held by each slot (n: none)
+  C key[VPT];    // (best, vertex) of an unvisited slot, or C::none()
   uint32_t vis = 0;
 #pragma unroll
   for (int s = 0; s < VPT; ++s) {
@@ -301,7 +331,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
 #pragma unroll
     for (int k = 0; k < DKS; ++k) xr[s][k] = 0;
     if (j < n) {
-      if (MODE == 0) {
+      if (PTS) {
 #pragma unroll
         for (int k = 0; k < DKS; ++k) {
           const Real v = k < d ? xg[(size_t)j * d + k] : Real(0);
@@ -313,6 +343,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     } else {
       vis |= 1u << s;
     }
+    key[s] = j < n && j != 0 ? C::make(Real(INFINITY), j) : C::none();
     Real nn = 0;
 #pragma unroll
     for (int k = 0; k < DKS; ++k) nn = fma(xr[s][k], xr[s][k], nn);
@@ -324,7 +355,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   // window-wide largest norm.  With every squared norm <= 2^100 (fp32) no
   // pair can overflow, so skipping pairs never hides an overflow.
   Real nmax = 0;
-  if (MODE == 0) {
+  if (PTS) {
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
       const int j = tid + s * T;
@@ -335,9 +366,9 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     }
   }
   // Non-negative IEEE values order like their bit patterns.
-  if (MODE == 0) atomicMax(&sh_nmax, (unsigned long long)BitsOfReal<Real>::get(nmax));
+  if (PTS) atomicMax(&sh_nmax, (unsigned long long)BitsOfReal<Real>::get(nmax));
   __syncthreads();
-  const bool gram = MODE == 0 && a.prefilter && BitsOfReal<Real>::put(sh_nmax) <= GramLimit<Real>::value;
+  const bool gram = MODE == 2 && BitsOfReal<Real>::put(sh_nmax) <= GramLimit<Real>::value;
   // Bound on |a - r| for a = fl(|x_j|^2 + |x_c|^2 - 2 x_j.x_c) (FMA chains of
   // DK terms) and r the exact real squared distance: 2.03 (DK + 3) u
   // (n_j + n_c) is enough; E uses 2 (DK + 4) u and the test below doubles it.
@@ -348,7 +379,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   PackRec<Real>* pk = static_cast<PackRec<Real>*>(a.pack) + base;
   for (int step = 0; step < n - 1; ++step) {
     Real xc[DKS];
-    if (MODE == 0) {
+    if (PTS) {
       if constexpr (XLDS) {
 #pragma unroll
         for (int k = 0; k < DKS; ++k) xc[k] = xs[cur * DK + k];
@@ -361,7 +392,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         for (int k = 0; k < DKS; ++k) xc[k] = sl[k];
       }
     }
-    if (MODE == 0 && a.repack && step > 0 && step % a.repack == 0) {
+    if (PTS && a.repack && step > 0 && step % a.repack == 0) {
 #pragma unroll
       for (int s = 0; s < VPT; ++s) {
         if (!((vis >> s) & 1u)) {
@@ -396,9 +427,11 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
             xr[s][k] = XLDS ? xs[j * DK + k] : (k < d ? xg[(size_t)j * d + k] : Real(0));
           nr[s] = gram ? xn[j] : Real(0);
           if (WSLOT) owner[j] = (uint8_t)wave;
+          key[s] = C::make(r.best, j);
         } else {
           vid[s] = n;
           vis |= 1u << s;
+          key[s] = C::none();
         }
       }
     }
@@ -428,7 +461,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       }
       if (__ballot(need)) {
         const int j = vid[s];
-        if (MODE == 0) {
+        if (PTS) {
           Real tot = 0;
 #pragma unroll
           for (int k = 0; k < DKS; ++k) {
@@ -446,6 +479,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
               best[s] = w;
               src[s] = cur;
               lim[s] = square_bound(w);
+              key[s] = C::make(w, j);
             }
           }
         } else {
@@ -455,13 +489,12 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
           if (live && w < best[s]) {
             best[s] = w;
             src[s] = cur;
+            key[s] = C::make(w, j);
           }
         }
       }
-      if (!((vis >> s) & 1u)) {
-        const C o = C::make(best[s], vid[s]);
-        if (o.less(c)) c = o;
-      }
+      // Visited slots hold C::none(), above every candidate.
+      c = cand_pick(key[s].less(c), key[s], c);
     }
     c = wave_min(c);
     if constexpr (WSLOT) {
@@ -502,6 +535,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         // graph is degenerate (only possible with non-finite weights).
         if ((vis >> s) & 1u) bad |= 1;
         vis |= 1u << s;
+        key[s] = C::none();
         const Real gw = g.weight();
         const size_t o = (size_t)base + step;
         a.edge_u[o] = gw < rmax ? src[s] : 0;
@@ -522,20 +556,6 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
 template <typename Real>
 __device__ __forceinline__ bool cand_eq(const Cand<Real>& x, const Cand<Real>& y) {
   return !x.less(y) && !y.less(x);
-}
-
-// c ? x : y, field by field (a select of whole structs goes through
-// scratch memory for the padded fp64 candidate).
-__device__ __forceinline__ Cand<float> cand_pick(bool c, const Cand<float>& x, const Cand<float>& y) {
-  Cand<float> r;
-  r.k = c ? x.k : y.k;
-  return r;
-}
-__device__ __forceinline__ Cand<double> cand_pick(bool c, const Cand<double>& x, const Cand<double>& y) {
-  Cand<double> r;
-  r.w = c ? x.w : y.w;
-  r.j = c ? x.j : y.j;
-  return r;
 }
 
 // Two smallest candidates seen so far (first < second).  Value selects only:

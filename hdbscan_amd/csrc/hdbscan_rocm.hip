@@ -456,6 +456,10 @@ int launch_prim_spec(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs
 template <typename Real, int VPT, int DK>
 int launch_prim_x(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
   if constexpr (VPT * DK * sizeof(Real) <= 64 * 4) {
+    if (a.prefilter) {
+      if (P.xlds) return launch_prim_one<Real, VPT, DK, 2, true>(s, P, batch, a);
+      return launch_prim_one<Real, VPT, DK, 2, false>(s, P, batch, a);
+    }
     if (P.xlds) return launch_prim_one<Real, VPT, DK, 0, true>(s, P, batch, a);
     return launch_prim_one<Real, VPT, DK, 0, false>(s, P, batch, a);
   }
