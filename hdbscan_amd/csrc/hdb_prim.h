@@ -319,6 +319,9 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   int src[VPT];
   int vid[VPT];  // vertex held by each slot (n: none)
   C key[VPT];    // (best, vertex) of an unvisited slot, or C::none()
+  // Liveness floor: the core distance (-inf with unit core distances) of an
+  // unvisited slot, +inf once visited or empty.
+  Real crx[VPT];
   uint32_t vis = 0;
 #pragma unroll
   for (int s = 0; s < VPT; ++s) {
@@ -344,6 +347,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       vis |= 1u << s;
     }
     key[s] = j < n && j != 0 ? C::make(Real(INFINITY), j) : C::none();
+    crx[s] = j < n && j != 0 ? (unit ? Real(-INFINITY) : cr[s]) : Real(INFINITY);
     Real nn = 0;
 #pragma unroll
     for (int k = 0; k < DKS; ++k) nn = fma(xr[s][k], xr[s][k], nn);
@@ -376,6 +380,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
 
   const Real rmax = RealTraits<Real>::max_value();
   int cur = 0, par = 0, bad = 0, pp = 0;
+  uint64_t bad_lanes = 0;  // lanes that formed a non-finite weight (wave-uniform)
   PackRec<Real>* pk = static_cast<PackRec<Real>*>(a.pack) + base;
   for (int step = 0; step < n - 1; ++step) {
     Real xc[DKS];
@@ -428,14 +433,17 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
           nr[s] = gram ? xn[j] : Real(0);
           if (WSLOT) owner[j] = (uint8_t)wave;
           key[s] = C::make(r.best, j);
+          crx[s] = unit ? Real(-INFINITY) : cr[s];
         } else {
           vid[s] = n;
           vis |= 1u << s;
           key[s] = C::none();
+          crx[s] = Real(INFINITY);
         }
       }
     }
     const Real ccur = unit ? Real(0) : cs[cur];
+    const Real ccx = unit ? Real(-INFINITY) : ccur;
     const Real ncur = gram ? xn[cur] : Real(0);
     C c = C::make(rmax, 0);
 #pragma unroll
@@ -443,10 +451,12 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
       // Only a weight strictly below best can change this vertex.  With core
       // distances, max(core_cur, core_j) >= best rules it out before any
       // distance is formed (w = max(core_cur, core_j, d) >= best).
-      // As one comparison: visited slots compare +inf, and with unit core
-      // distances -inf keeps every unvisited vertex live.
-      const Real lv = ((vis >> s) & 1u) ? Real(INFINITY) : unit ? Real(-INFINITY) : fmax(ccur, cr[s]);
+      // As one comparison against the slot's floor (+inf once visited, -inf
+      // with unit core distances, which keeps every unvisited vertex live).
+      // Core distances are never NaN here (non-finite input is rejected).
+      const Real lv = ccx > crx[s] ? ccx : crx[s];
       const bool live = lv < best[s];
+      const uint64_t live_lanes = __builtin_amdgcn_ballot_w64(lv < best[s]);
       bool need = live;
       if (gram && __ballot(live)) {
         // Certified skip: a - 2E >= lim (1 + 2^-16) implies the exact
@@ -459,7 +469,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         const Real err = gram_k * sn;
         need = live && !(approx - Real(2) * err >= lim[s] * Real(1.0000152587890625));
       }
-      if (__ballot(need)) {
+      if (MODE == 2 ? __ballot(need) != 0 : live_lanes != 0) {
         const int j = vid[s];
         if (PTS) {
           Real tot = 0;
@@ -468,7 +478,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
             const Real dl = xr[s][k] - xc[k];
             tot = tot + dl * dl;
           }
-          bad |= live && !isfinite(tot);
+          bad_lanes |= __builtin_amdgcn_ballot_w64(!isfinite(tot)) & live_lanes;
           // tot >= lim[s] (an upper bound of best^2) implies sqrt(tot) >= best:
           // no update, so the correctly rounded square root is skipped.
           const bool near = tot < (live ? lim[s] : Real(-INFINITY));
@@ -484,7 +494,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
           }
         } else {
           const Real dd = live ? dg[(size_t)cur * n + j] : Real(0);
-          bad |= live && !isfinite(dd);
+          bad_lanes |= __builtin_amdgcn_ballot_w64(!isfinite(dd)) & live_lanes;
           const Real w = unit ? dd : fmax(fmax(ccur, cr[s]), dd);
           if (live && w < best[s]) {
             best[s] = w;
@@ -536,6 +546,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         if ((vis >> s) & 1u) bad |= 1;
         vis |= 1u << s;
         key[s] = C::none();
+        crx[s] = Real(INFINITY);
         const Real gw = g.weight();
         const size_t o = (size_t)base + step;
         a.edge_u[o] = gw < rmax ? src[s] : 0;
@@ -545,7 +556,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     }
     cur = nxt;
   }
-  if (bad) atomicOr(&sh_err, 1);
+  if (bad || (bad_lanes && lane == 0)) atomicOr(&sh_err, 1);
   __syncthreads();
   if (tid == 0) {
     a.wflag[b] = sh_err;
