@@ -113,7 +113,7 @@ template <int DK, int RSET> struct GramShape {
   static constexpr int KP = (3 * DK + 31) / 32 * 32;    // three bf16 segments, MFMA K = 32
   static constexpr int NQ = KP / 32;                    // MFMAs per 16x16 tile
   static constexpr int KS = KP + 8;                     // LDS column stride (bf16)
-  static constexpr int TJ = DK >= 128 ? 32 : 64;        // columns per chunk
+  static constexpr int TJ = DK >= 128 ? 32 : DK <= 32 ? 128 : 64;  // columns per chunk
   static constexpr int RS = RSET;
   static constexpr int WR = 16 * RS;                    // rows per wave
   static constexpr int MAXT = DK >= 64 ? 256 : 512;     // threads per workgroup
