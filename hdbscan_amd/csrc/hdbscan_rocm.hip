@@ -508,7 +508,7 @@ template <typename Real>
 int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_dist, const Real* given_core,
                 const Launch& L, int64_t d, long long mcs, int ms, bool unit, double eps, int sort_depth,
                 int32_t* labels, unsigned char* ws, int32_t* status, int32_t* exp_i, void* exp_r,
-                bool gram_prep = true) {
+                bool gram_prep = true, hipEvent_t core_done = nullptr) {
   int32_t* eu = reinterpret_cast<int32_t*>(ws + P.off_eu);
   int32_t* ev = reinterpret_cast<int32_t*>(ws + P.off_ev);
   Real* ew = reinterpret_cast<Real*>(ws + P.off_ew);
@@ -533,6 +533,7 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
       stage_mark(s, P.core_gram ? "core_exact_kernel" : P.core_topk ? "core_topk_kernel" : "core_select_kernel");
     }
   }
+  if (core_done && hipEventRecord(core_done, s) != hipSuccess) return fail("event record failed");
   PrimArgs pa;
   pa.x = x;
   pa.dist = dist;
@@ -677,7 +678,7 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
 struct SideStreams {
   bool ready = false;
   hipStream_t s[kSideStreams];
-  hipEvent_t fork, join[kSideStreams];
+  hipEvent_t fork, head, join[kSideStreams];
 };
 std::mutex g_side_mutex;
 SideStreams g_side[64];
@@ -691,7 +692,9 @@ int side_streams(SideStreams** out) {
       if (hipStreamCreateWithFlags(&S.s[i], hipStreamNonBlocking) != hipSuccess ||
           hipEventCreateWithFlags(&S.join[i], hipEventDisableTiming) != hipSuccess)
         return fail("cannot create internal streams");
-    if (hipEventCreateWithFlags(&S.fork, hipEventDisableTiming) != hipSuccess) return fail("cannot create events");
+    if (hipEventCreateWithFlags(&S.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S.head, hipEventDisableTiming) != hipSuccess)
+      return fail("cannot create events");
     S.ready = true;
   }
   *out = &S;
@@ -715,14 +718,22 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
     }
   }
   if (!err && hipEventRecord(S->fork, s) != hipSuccess) err = fail("event record failed");
-  // Largest buckets first so that they start earliest.
+  // The bucket of the largest windows is the critical path (its Prim runs
+  // longest): it gets the first internal stream to itself, and the other
+  // buckets, spread over the remaining streams, start once its core
+  // distances are done, so that its first stages have the whole device.
   int used = 0;
+  bool waited[kSideStreams] = {false, false, false, false};
   for (int k = kBuckets - 1; k >= 0 && !err; --k) {
     if (!R.used[k]) continue;
-    hipStream_t t = S->s[used % kSideStreams];
-    if (used < kSideStreams && hipStreamWaitEvent(t, S->fork, 0) != hipSuccess) {
-      err = fail("stream wait failed");
-      break;
+    const int si = used == 0 ? 0 : 1 + (used - 1) % (kSideStreams - 1);
+    hipStream_t t = S->s[si];
+    if (!waited[si]) {
+      if (hipStreamWaitEvent(t, si == 0 ? S->fork : S->head, 0) != hipSuccess) {
+        err = fail("stream wait failed");
+        break;
+      }
+      waited[si] = true;
     }
     Launch L;
     L.win.off = off_dev;
@@ -733,10 +744,11 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
     L.points = R.points;
     L.nmax = R.nmax[k];
     err = run_cluster<Real>(t, R.plan[k], x, nullptr, nullptr, L, d, mcs, ms, ms <= 2, eps, -100000, labels, ws,
-                            status, nullptr, nullptr, false);
+                            status, nullptr, nullptr, false, used == 0 ? S->head : nullptr);
     ++used;
   }
-  for (int i = 0; i < std::min(used, kSideStreams); ++i) {
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (!waited[i]) continue;
     if (hipEventRecord(S->join[i], S->s[i]) != hipSuccess || hipStreamWaitEvent(s, S->join[i], 0) != hipSuccess)
       err = err ? err : fail("stream join failed");
   }
