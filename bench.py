@@ -310,6 +310,14 @@ def main():
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
                     "algorithmic": f"{algo_bytes:.4g} B per launch (N(N-1)/2 weights per window)"}
+    elif dominant == "core_select_kernel":
+        # Selection from the materialised matrix: every row read once.
+        itemsize = 4 if precision == "float32" else 8
+        algo_bytes = batch * n * n * itemsize
+        achieved = algo_bytes / (stages[dominant] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                    "algorithmic": f"{algo_bytes:.4g} B per launch (N^2 weights per window)"}
     elif dominant in ("prim_kernel", "core_topk_kernel", "distance_kernel"):
         peak = FP32_PEAK_TFLOPS if precision == "float32" else FP64_PEAK_TFLOPS
         achieved = dist_flops / (stages[dominant] * 1e-3) / 1e12

@@ -270,12 +270,34 @@ __global__ __launch_bounds__(256) void core_select_list_kernel(const Real* dist,
   Real lst[KMAX];
 #pragma unroll
   for (int q = 0; q < KMAX; ++q) lst[q] = Real(INFINITY);
-  for (int j = lane; j < n; j += 64) {
-    const Real v = src[j];
-    if (v < lst[KMAX - 1]) {
+  if constexpr (sizeof(Real) == 4) {
+    // Eight coalesced loads in flight per lane, then a branch-free insertion:
+    // with lst ascending, the new q-th smallest of lst and v is the median of
+    // (lst[q-1], lst[q], v) (one instruction), the smallest a compare-select.
+    // Distances are never NaN here (non-finite input is rejected); columns
+    // past the row read as +inf and change nothing.
+    for (int j0 = 0; j0 < n; j0 += 8 * 64) {
+      float v[8];
 #pragma unroll
-      for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
-      lst[0] = fmin(lst[0], v);
+      for (int u = 0; u < 8; ++u) {
+        const int j = j0 + u * 64 + lane;
+        v[u] = j < n ? src[j] : INFINITY;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+#pragma unroll
+        for (int q = KMAX - 1; q > 0; --q) lst[q] = __builtin_amdgcn_fmed3f(lst[q - 1], lst[q], v[u]);
+        lst[0] = v[u] < lst[0] ? v[u] : lst[0];
+      }
+    }
+  } else {
+    for (int j = lane; j < n; j += 64) {
+      const Real v = src[j];
+      if (v < lst[KMAX - 1]) {
+#pragma unroll
+        for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
+        lst[0] = fmin(lst[0], v);
+      }
     }
   }
   int cnt = 0;

@@ -530,7 +530,7 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
     if (!unit) {
       if (int r = launch_core<Real>(s, P, x, L, d, ms, core, matrix, ws, status, matrix_ready, gram_prep)) return r;
       core_in = core;
-      stage_mark(s, P.core_gram ? "core_exact_kernel" : P.core_topk ? "core_topk_kernel" : "core_select_kernel");
+      stage_mark(s, P.core_gram ? "core_exact_kernel" : matrix_ready ? "core_select_kernel" : "core_topk_kernel");
     }
   }
   if (core_done && hipEventRecord(core_done, s) != hipSuccess) return fail("event record failed");
