@@ -118,9 +118,17 @@ __device__ __forceinline__ void topk_offer(Real* lst, Real& lim, Real tot) {
   if (tot < lim) {
     const Real v = sqrt(tot);
     if (v < lst[KMAX - 1]) {
+      if constexpr (sizeof(Real) == 4) {
+        // Ascending list: the new q-th smallest is the median of
+        // (lst[q-1], lst[q], v); v is finite (tot < lim).
 #pragma unroll
-      for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
-      lst[0] = fmin(lst[0], v);
+        for (int q = KMAX - 1; q > 0; --q) lst[q] = __builtin_amdgcn_fmed3f(lst[q - 1], lst[q], v);
+        lst[0] = v < lst[0] ? v : lst[0];
+      } else {
+#pragma unroll
+        for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
+        lst[0] = fmin(lst[0], v);
+      }
       lim = sq_bound(lst[KMAX - 1]);
     }
   }
