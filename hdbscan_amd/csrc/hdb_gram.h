@@ -70,9 +70,11 @@ __device__ __forceinline__ float row16_min(float v) {
 template <int K>
 __device__ __forceinline__ void list_offer(float* lst, float v) {
   if (v < lst[K - 1]) {
+    // Ascending list, v not NaN (it compared below lst[K-1]): the new q-th
+    // smallest is the median of (lst[q-1], lst[q], v).
 #pragma unroll
-    for (int q = K - 1; q > 0; --q) lst[q] = fminf(lst[q], fmaxf(lst[q - 1], v));
-    lst[0] = fminf(lst[0], v);
+    for (int q = K - 1; q > 0; --q) lst[q] = __builtin_amdgcn_fmed3f(lst[q - 1], lst[q], v);
+    lst[0] = v < lst[0] ? v : lst[0];
   }
 }
 
