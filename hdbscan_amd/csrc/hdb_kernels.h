@@ -299,12 +299,22 @@ __global__ __launch_bounds__(256) void core_select_list_kernel(const Real* dist,
       }
     }
   } else {
-    for (int j = lane; j < n; j += 64) {
-      const Real v = src[j];
-      if (v < lst[KMAX - 1]) {
+    // fp64 (no median-of-three instruction): eight loads in flight, then the
+    // guarded insertion.
+    for (int j0 = 0; j0 < n; j0 += 8 * 64) {
+      Real v[8];
 #pragma unroll
-        for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v));
-        lst[0] = fmin(lst[0], v);
+      for (int u = 0; u < 8; ++u) {
+        const int j = j0 + u * 64 + lane;
+        v[u] = j < n ? src[j] : Real(INFINITY);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (v[u] < lst[KMAX - 1]) {
+#pragma unroll
+          for (int q = KMAX - 1; q > 0; --q) lst[q] = fmin(lst[q], fmax(lst[q - 1], v[u]));
+          lst[0] = fmin(lst[0], v[u]);
+        }
       }
     }
   }
