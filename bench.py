@@ -212,9 +212,9 @@ def main():
                          "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box here has one GPU, so --gpus {args.gpus} is not run. "
-              "Windows are independent; the multi-GPU design (windows sharded across GPUs, no exchange) "
-              "is described in DESIGN.md.", file=sys.stderr)
+        print(f"bench.py measures one GPU, so --gpus {args.gpus} is not run.  Windows are independent: "
+              "on several GPUs each rank would take its own share of the batch (one process per GPU, no "
+              "exchange between ranks).", file=sys.stderr)
         sys.exit(2)
 
     import torch
