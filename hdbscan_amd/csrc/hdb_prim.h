@@ -182,6 +182,40 @@ __device__ __forceinline__ Cand<Real> wave_min(Cand<Real> c) {
   return lane_cand(c, 63);
 }
 
+// DPP move whose unwritten lanes hold all ones, the identity of an unsigned
+// minimum: with that immediate as the old value the move folds into the min
+// that consumes it (one v_min_u32 with a DPP source per level).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_umin_src(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+// Wave-wide unsigned minimum of a 32-bit value, the same lane pattern as
+// wave_min.
+__device__ __forceinline__ uint32_t wave_umin32(uint32_t v) {
+  v = min(v, dpp_umin_src<kDppXor1, 0xF>(v));
+  v = min(v, dpp_umin_src<kDppXor2, 0xF>(v));
+  v = min(v, dpp_umin_src<kDppRor4, 0xF>(v));
+  v = min(v, dpp_umin_src<kDppRor8, 0xF>(v));
+  v = min(v, dpp_umin_src<kDppBcast15, 0xA>(v));
+  v = min(v, dpp_umin_src<kDppBcast31, 0xC>(v));
+  return read_lane(v, 63);
+}
+
+// The fp32 candidate key in two 32-bit passes: the smallest weight pattern,
+// then the smallest vertex among the lanes holding it.  That is the
+// lexicographic minimum of (weight, vertex), the 64-bit key minimum, in 12
+// single-instruction levels instead of six levels of 64-bit compares and
+// selects.
+__device__ __forceinline__ Cand<float> wave_min(Cand<float> c) {
+  const uint32_t w = (uint32_t)(c.k >> 32);
+  const uint32_t wm = wave_umin32(w);
+  const uint32_t jm = wave_umin32(w == wm ? (uint32_t)c.k : 0xFFFFFFFFu);
+  Cand<float> r;
+  r.k = ((uint64_t)wm << 32) | jm;
+  return r;
+}
+
 // Minimum over lanes 0..15, returned wave-uniform (the other rows are not
 // read).  All 64 lanes must be active.
 template <typename Real>
@@ -380,6 +414,8 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
 
   const Real rmax = RealTraits<Real>::max_value();
   int cur = 0, par = 0, bad = 0, pp = 0;
+  // Countdown to the next repacking, at steps repack, 2 repack, ... (0: never).
+  int to_repack = a.repack ? a.repack + 1 : 0;
   uint64_t bad_lanes = 0;  // lanes that formed a non-finite weight (wave-uniform)
   PackRec<Real>* pk = static_cast<PackRec<Real>*>(a.pack) + base;
   for (int step = 0; step < n - 1; ++step) {
@@ -397,7 +433,8 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         for (int k = 0; k < DKS; ++k) xc[k] = sl[k];
       }
     }
-    if (PTS && a.repack && step > 0 && step % a.repack == 0) {
+    if (PTS && to_repack && --to_repack == 0) {
+      to_repack = a.repack;
 #pragma unroll
       for (int s = 0; s < VPT; ++s) {
         if (!((vis >> s) & 1u)) {
