@@ -223,6 +223,25 @@ __device__ __forceinline__ Cand<Real> row0_min(Cand<Real> c) {
   return lane_cand(row_min(c), 0);
 }
 
+// Unsigned minimum over each 16-lane row, read from lane 0.
+__device__ __forceinline__ uint32_t row0_umin32(uint32_t v) {
+  v = min(v, dpp_umin_src<kDppXor1, 0xF>(v));
+  v = min(v, dpp_umin_src<kDppXor2, 0xF>(v));
+  v = min(v, dpp_umin_src<kDppRor4, 0xF>(v));
+  v = min(v, dpp_umin_src<kDppRor8, 0xF>(v));
+  return read_lane(v, 0);
+}
+
+// fp32 keys over lanes 0..15 in two 32-bit passes, as wave_min.
+__device__ __forceinline__ Cand<float> row0_min(Cand<float> c) {
+  const uint32_t w = (uint32_t)(c.k >> 32);
+  const uint32_t wm = row0_umin32(w);
+  const uint32_t jm = row0_umin32(w == wm ? (uint32_t)c.k : 0xFFFFFFFFu);
+  Cand<float> r;
+  r.k = ((uint64_t)wm << 32) | jm;
+  return r;
+}
+
 // Unit roundoff of Real.
 template <typename Real> struct Unit;
 template <> struct Unit<float> { static constexpr float u = 5.9604644775390625e-08f; };  // 2^-24

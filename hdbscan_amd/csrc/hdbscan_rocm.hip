@@ -221,9 +221,11 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.core_dk = pad_dims(d);
   // Points path: owned coordinates must fit 64 VGPRs per thread.
   // Vertices per Prim thread: about 512 threads per window measured best at
-  // N=1024 (fewer waves to synchronise per step, two slots of ILP); larger
-  // windows grow the block up to 1024 threads when registers would not hold
-  // the coordinates of more slots.
+  // N=1024 without core distances (fewer waves to synchronise per step, two
+  // slots of ILP), 1024 threads with core distances and at most 16 fp32
+  // coordinates per point (1.5 % faster at N=1024, D=16); larger windows
+  // grow the block up to 1024 threads when registers would not hold the
+  // coordinates of more slots.
   const int dk = pad_dims(d);
   P.dk = dk;
   const int want_vpt = (flags >> 8) & 15;
@@ -232,6 +234,7 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   int vpt = want_vpt == 8 ? 0 : want_vpt;
   if (!vpt) {
     vpt = n <= 512 ? 1 : n <= 1024 ? 2 : 4;
+    if (vpt == 2 && P.core && dk > 0 && (int64_t)dk * rs <= 64) vpt = 1;
     while (vpt > 1 && (int64_t)vpt * dk * rs > 64 * 4 && (int64_t)(vpt / 2) * 1024 >= n) vpt /= 2;
   }
   while ((int64_t)vpt * 1024 < n) vpt *= 2;
