@@ -461,8 +461,8 @@ __global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
     for (int i = tid; i < n; i += T) xr[2 * n + i] = i < nl - n ? t.birth[i] : Real(0);
   }
   __syncthreads();
+  if (tid < 64) select_clusters_wave(t, n, nl, (Real)a.epsilon);
   if (tid == 0) {
-    select_clusters(t, n, nl, (Real)a.epsilon);
     clk[5] = wall_clock64();
     for (int q = 0; q < 6; ++q) sh_clk[q] = clk[q];
   }

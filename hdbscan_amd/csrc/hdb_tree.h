@@ -148,6 +148,13 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Orders one wave's LDS accesses: those after it see the wave's earlier
+// writes (a wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Partition of keys[lo..hi] by one full wave; returns the pivot's final index.
 __device__ __forceinline__ float lane0_value(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
@@ -766,64 +773,132 @@ __device__ void condense_points(const TreeView<Real>& t, int n, const int16_t* p
   }
 }
 
-// Excess-of-mass selection, epsilon merge and label resolution.  Leaves
-// resolved()[c - n] = label of cluster c (or -1).
+// Excess-of-mass selection, epsilon merge and label resolution, run by one
+// wave level by level over the cluster tree.  Leaves resolved()[c - n] =
+// label of cluster c (or -1).  Cluster ids are relative (c = id - n, root 0);
+// the children of a split are consecutive, the first at an odd id, so every
+// cluster has no child or two (c1, c1 + 1).  Sibling clusters never depend
+// on each other, so each pass handles one depth level at a time, in the
+// order the serial reference passes imply:
+//   EOM (reference hdbscan_gpu/_kernels.py:685-699): deepest level first,
+//     sub = (0 + stab(c1)) + stab(c2) (0 for a leaf); sub > stab deselects
+//     and replaces stab;
+//   suppression under a selected ancestor (:700-709), the epsilon stop
+//     (nearest ancestor with 1/birth > eps, else the root's child on the
+//     path) and the chosen marks (:716-739): root side first;
+//   label numbering in ascending id order (:740-758), then each cluster's
+//     label or its parent's, root side first.
+// A level reads what the previous level wrote (wave_sync between levels).
 template <typename Real>
-__device__ void select_clusters(const TreeView<Real>& t, int n, int next_label, Real epsilon) {
-  int16_t* first_child = t.first_child();
-  int16_t* next_sibling = t.next_sibling();
-  int16_t* blocked = t.blocked();
+__device__ void select_clusters_wave(const TreeView<Real>& t, int n, int next_label, Real epsilon) {
+  const int lane = threadIdx.x & 63;
   const int count = next_label - n;
-  for (int c = 0; c < count; ++c) {
-    first_child[c] = -1;
-    next_sibling[c] = -1;
-  }
-  // Descending insertion keeps each child list in ascending ID order.
-  for (int child = next_label - 1; child > n; --child) {
-    const int p = t.parent[child - n];
-    next_sibling[child - n] = first_child[p - n];
-    first_child[p - n] = (int16_t)child;
-  }
-  for (int node = next_label - 1; node > n; --node) {
-    Real sub = 0;
-    for (int ch = first_child[node - n]; ch != -1; ch = next_sibling[ch - n]) sub += t.stab[ch - n];
-    if (sub > t.stab[node - n]) {
-      t.selected[node - n] = 0;
-      t.stab[node - n] = sub;
-    }
-  }
-  // Parents carry smaller IDs: one forward pass suppresses every descendant
-  // of a selected cluster.
-  blocked[0] = 0;
-  for (int node = n + 1; node < next_label; ++node) {
-    const int inherited = blocked[t.parent[node - n] - n];
-    blocked[node - n] = (int16_t)(inherited || t.selected[node - n]);
-    if (inherited) t.selected[node - n] = 0;
-  }
-  if (epsilon > Real(0)) {
-    int16_t* stop = t.stop();
-    int16_t* mark = t.mark();
-    for (int c = 0; c < count; ++c) mark[c] = 0;
-    for (int node = n + 1; node < next_label; ++node) {
-      const int anc = t.parent[node - n];
-      // Ancestor test is strict ">", start test strict "<": they differ at
-      // equality on purpose.
-      stop[node - n] = (int16_t)(anc == n ? node
-                                          : (Real(1) / t.birth[anc - n] > epsilon ? anc : stop[anc - n]));
-      if (!t.selected[node - n]) continue;
-      const int chosen = Real(1) / t.birth[node - n] < epsilon ? stop[node - n] : node;
-      mark[chosen - n] = 1;
-    }
-    for (int node = n + 1; node < next_label; ++node) t.selected[node - n] = (uint8_t)(mark[node - n] == 1);
-  }
+  int16_t* child = t.first_child();  // first child, or -1
+  int16_t* order = t.next_sibling(); // clusters by depth, root first
+  int16_t* blocked = t.blocked();
+  int16_t* stop = t.stop();
+  int16_t* mark = t.mark();
   int16_t* label_map = t.label_map();
   int16_t* resolved = t.resolved();
+  // The sort pairs are dead after condensation: they hold the level ends.
+  int16_t* level_end = reinterpret_cast<int16_t*>(t.keys);
+  for (int c = lane; c < count; c += 64) {
+    child[c] = -1;
+    mark[c] = 0;
+  }
+  wave_sync();
+  for (int c = 1 + 2 * lane; c < count; c += 128) child[t.parent[c] - n] = (int16_t)c;
+  if (lane == 0) {
+    order[0] = 0;
+    level_end[0] = 1;
+  }
+  wave_sync();
+  // Breadth-first levels: the children of level k-1 (in its order) form k.
+  int levels = 1, lo = 0, hi = 1;
+  while (lo < hi) {
+    int end = hi;
+    for (int i0 = lo; i0 < hi; i0 += 64) {
+      const int i = i0 + lane;
+      const int ch = i < hi ? child[order[i]] : -1;
+      const uint64_t m = __ballot(ch >= 0);
+      if (ch >= 0) {
+        const int q = end + 2 * (int)lanes_below(m);
+        order[q] = (int16_t)ch;
+        order[q + 1] = (int16_t)(ch + 1);
+      }
+      end += 2 * (int)__popcll(m);
+    }
+    wave_sync();
+    if (end > hi) {
+      if (lane == 0) level_end[levels] = (int16_t)end;
+      ++levels;
+    }
+    lo = hi;
+    hi = end;
+    wave_sync();
+  }
+  // Excess of mass, deepest level first (the root is never selected).
+  for (int k = levels - 1; k >= 1; --k) {
+    const int b0 = level_end[k - 1], b1 = level_end[k];
+    for (int i = b0 + lane; i < b1; i += 64) {
+      const int c = order[i];
+      const int ch = child[c];
+      const Real sub = ch < 0 ? Real(0) : (Real(0) + t.stab[ch]) + t.stab[ch + 1];
+      if (sub > t.stab[c]) {
+        t.selected[c] = 0;
+        t.stab[c] = sub;
+      }
+    }
+    wave_sync();
+  }
+  // Suppression, epsilon stop and marks, root side first.
+  if (lane == 0) blocked[0] = 0;
+  wave_sync();
+  for (int k = 1; k < levels; ++k) {
+    const int b0 = level_end[k - 1], b1 = level_end[k];
+    for (int i = b0 + lane; i < b1; i += 64) {
+      const int c = order[i];
+      const int p = t.parent[c] - n;
+      const bool inherited = blocked[p] != 0;
+      const bool sel = !inherited && t.selected[c];
+      blocked[c] = (int16_t)(inherited || t.selected[c]);
+      if (inherited) t.selected[c] = 0;
+      if (epsilon > Real(0)) {
+        // Ancestor test is strict ">", start test strict "<": they differ at
+        // equality on purpose.
+        const int st = p == 0 ? c + n : (Real(1) / t.birth[p] > epsilon ? p + n : stop[p]);
+        stop[c] = (int16_t)st;
+        if (sel) {
+          const int chosen = Real(1) / t.birth[c] < epsilon ? st : c + n;
+          mark[chosen - n] = 1;
+        }
+      }
+    }
+    wave_sync();
+  }
+  if (epsilon > Real(0)) {
+    for (int c = 1 + lane; c < count; c += 64) t.selected[c] = (uint8_t)(mark[c] == 1);
+    wave_sync();
+  }
+  // Labels in ascending id order.
   int label = 0;
-  for (int node = n + 1; node < next_label; ++node)
-    label_map[node - n] = t.selected[node - n] ? (int16_t)label++ : (int16_t)-1;
-  resolved[0] = -1;
-  for (int node = n + 1; node < next_label; ++node)
-    resolved[node - n] = t.selected[node - n] ? label_map[node - n] : resolved[t.parent[node - n] - n];
+  for (int c0 = 1; c0 < count; c0 += 64) {
+    const int c = c0 + lane;
+    const bool sel = c < count && t.selected[c];
+    const uint64_t m = __ballot(sel);
+    if (c < count) label_map[c] = sel ? (int16_t)(label + (int)lanes_below(m)) : (int16_t)-1;
+    label += (int)__popcll(m);
+  }
+  if (lane == 0) resolved[0] = -1;
+  wave_sync();
+  for (int k = 1; k < levels; ++k) {
+    const int b0 = level_end[k - 1], b1 = level_end[k];
+    for (int i = b0 + lane; i < b1; i += 64) {
+      const int c = order[i];
+      resolved[c] = t.selected[c] ? label_map[c] : resolved[t.parent[c] - n];
+    }
+    wave_sync();
+  }
 }
 
 }  // namespace hdb
