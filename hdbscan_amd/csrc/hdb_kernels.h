@@ -448,7 +448,7 @@ __global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
   __syncthreads();
   ClusterSlot<Real>* slots = reinterpret_cast<ClusterSlot<Real>*>(
       a.slots + (a.slot_stride ? (size_t)b * a.slot_stride : (size_t)p0 * sizeof(ClusterSlot<Real>)));
-  const int nl = condense_walks(t, n, slots, node_slot, t.parent, reinterpret_cast<int32_t*>(t.uf), sh_counters);
+  const int nl = condense_clusters(t, n, a.mcs, slots, node_slot, t.parent, sh_counters);
   if (tid == 0) clk[4] = wall_clock64();
   for (int i = tid; i < n - 1; i += T) {
     t.uf[t.left[i]] = (int16_t)(n + i);
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
   }
   if (tid == 0) t.uf[2 * n - 2] = (int16_t)(2 * n - 2);  // the root is its own parent
   __syncthreads();
-  condense_points(t, n, t.uf, node_slot, slots);
+  condense_points(t, n, t.uf, node_slot);
   if (xr) {
     for (int i = tid; i < n; i += T) xr[2 * n + i] = i < nl - n ? t.birth[i] : Real(0);
   }

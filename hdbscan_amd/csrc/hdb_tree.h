@@ -554,222 +554,328 @@ __device__ void condense_prepare(const TreeView<Real>& t, int n, long long mcs, 
   }
 }
 
-// Per-cluster record of the cluster walks (device memory, one per cluster).
+// Per-cluster record of the condensation (device memory, one per cluster,
+// indexed by cluster id - n).
 template <typename Real> struct ClusterSlot {
   Real birth;        // lambda at which the cluster appears (0 for the root)
-  Real stab;         // stability accumulated along the cluster's path
-  int32_t parent;    // parent slot (-1 for the root)
-  int32_t key;       // depth << 13 | preorder of the node that splits it, -1 if none
-  int32_t root;      // first node of the cluster's path
-  int32_t depth;     // depth of that node in the single-linkage tree
-  int32_t pre;       // its preorder index (left subtree first)
-  int32_t label;     // final cluster id
+  Real stab;         // unused (the stability stays in registers)
+  int32_t parent;    // parent cluster id (-1 for the root)
+  int32_t start;     // first entry of the cluster's path in the path arrays
+  int32_t len;       // nodes on the path
+  int32_t split_l;   // sizes of the children of the path's last node when it
+  int32_t split_r;   //   splits (0 otherwise)
+  int32_t pad;
 };
 
-// Condensation by cluster walks (after condense_prepare).  The nodes kept by the reference's
-// breadth-first condensation form, per cluster, one downward path: from the
-// cluster's first node, each node has at most one child of >= mcs points that
-// inherits the cluster, and the path ends at a node whose children are both
-// >= mcs (a split: two new clusters) or both smaller.  The reference adds to
-// a cluster's stability only while visiting that path, in path order, so a
-// lane can accumulate it alone: each smaller child adds lambda - birth once
-// per point (all its points leave at the same lambda), a split adds
-// (lambda - birth) * size for the left then the right child.  A child
-// cluster's walk can start once its parent's walk reaches the split.  The
-// reference numbers new clusters in breadth-first order of the split nodes,
-// i.e. by (depth, preorder) of those nodes: split number r gives its children
-// n+1+2r (left) and n+2+2r (right).  node_slot[i] records the slot of each
-// visited node n+i.
-// Called by every thread of the workgroup; returns next_label (uniform).
-template <typename Real>
-__device__ int condense_walks(const TreeView<Real>& t, int n, ClusterSlot<Real>* slots, int16_t* node_slot,
-                              const int16_t* pruned, int32_t* split_keys, int* counters) {
-  // counters: [2] slots allocated, [3] splits
-  const int tid = threadIdx.x, T = blockDim.x;
-  if (tid == 0) {
-    ClusterSlot<Real> r;
-    r.birth = 0;
-    r.stab = 0;
-    r.parent = -1;
-    r.key = -1;
-    r.root = 2 * n - 2;
-    r.depth = 0;
-    r.pre = 0;
-    r.label = n;
-    slots[0] = r;
+// Exclusive prefix sum of v[0..count-1] in place (count <= 4 * blockDim.x),
+// v[count] = total.  Called by every thread of the workgroup.
+__device__ void block_exclusive_scan(int16_t* v, int count, int* wave_sums) {
+  const int tid = threadIdx.x, T = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  const int per = (count + T - 1) / T;
+  const int lo = min(count, tid * per), hi = min(count, lo + per);
+  int sum = 0;
+  for (int i = lo; i < hi; ++i) sum += v[i];
+  int inc = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
   }
-  // One wave walks every cluster, taking clusters from a queue as soon as
-  // their parent's walk reaches its split (a level-by-level schedule would
-  // make the critical path the sum over levels of the longest walk instead
-  // of the longest root-to-leaf sum of walks).  A lane that reaches a split
-  // finishes its cluster, continues with the left child and queues the right
-  // one; idle lanes take queued clusters in queue order.  Walks are
-  // independent, so the order does not change any result.  Queue entries
-  // (split node | right child's slot << 13, and the split node's depth |
-  // preorder << 13) live in split_keys, free until the ranking below; slots
-  // are allocated in pairs from 1, so a slot's parity tells left from right.
-  if (tid < 64) {
-    const int lane = tid;
-    int32_t* queue = split_keys;
-    int qhead = 0, qtail = 0;  // queue entries taken / written (wave-uniform)
-    int nalloc = 1;            // slots allocated (wave-uniform)
-    bool active = lane == 0;
-    int s = 0, x = 2 * n - 2, d = 0, p = 0;
-    Real b = 0, stab = 0;
-    for (;;) {
-      const uint64_t idle = __ballot(!active);
-      const int avail = qtail - qhead;
-      if (avail > 0 && idle) {
-        const int r = (int)lanes_below(idle);
-        if (!active && r < avail) {
-          const int e0 = queue[2 * (qhead + r)], e1 = queue[2 * (qhead + r) + 1];
-          const int i = e0 & 0x1fff;
-          const int a = t.left[i];
-          const int sa = a < n ? 1 : t.sizes[a - n];
-          s = e0 >> 13;
-          x = t.right[i];
-          d = (e1 & 0x1fff) + 1;
-          p = (e1 >> 13) + 2 * sa;
-          b = t.keys[i].w;
-          stab = 0;
-          active = true;
-        }
-        qhead += min(avail, (int)__popcll(idle));
+  if (lane == 63) wave_sums[wave] = inc;
+  __syncthreads();
+  int run = inc - sum;
+  for (int w = 0; w < wave; ++w) run += wave_sums[w];
+  for (int i = lo; i < hi; ++i) {
+    const int x = v[i];
+    v[i] = (int16_t)run;
+    run += x;
+  }
+  if (hi == count && lo < hi) v[count] = (int16_t)run;
+  if (count == 0 && tid == 0) v[0] = 0;
+  __syncthreads();
+}
+
+// Condensation (after condense_prepare), in parallel over the internal nodes
+// n+i of the single-linkage tree.  The nodes kept by the reference's
+// breadth-first condensation are the root and every node of >= mcs points;
+// per cluster they form one downward path: from the cluster's first node
+// (the root, or a child of a split: a node whose two children both have
+// >= mcs points) each node's only child of >= mcs points inherits the
+// cluster, and the path ends at a split or at a node whose children are
+// both smaller.  The reference numbers new clusters in breadth-first order
+// of the split nodes, i.e. by (depth, preorder) of those nodes: split number
+// r gives its children n+1+2r (left) and n+2+2r (right).  It adds to a
+// cluster's stability only along that path, in path order: each smaller
+// child adds lambda - birth once per point (all its points leave at the same
+// lambda), a split adds (lambda - birth) * size for the left then the right
+// child -- so each cluster's sum is formed by one lane, in that order.
+//   1. parent of every internal node (scatter from left/right);
+//   2. depth and preorder of every kept node by pointer jumping (a child's
+//      preorder offset is 1 for the left child, 1 + nodes(left) = 2*size(left)
+//      for the right);
+//   3. the first node of each kept node's cluster by pointer jumping;
+//   4. split ranks by counting over the compacted (depth, preorder) keys,
+//      cluster ids, node_slot[i] = cluster id - n of each kept node n+i;
+//   5. path arrays (lambda and pruned count per node, in path order) and
+//      each cluster's sequential stability sum.
+// Scratch: uf, comp, edge_v, point_parent, birth and stability (all dead or
+// not yet written).  Leaves birth, stability, parent and selected per cluster
+// (indexed by cluster id - n).  Needs n - 1 <= 4 * blockDim.x.  Called by
+// every thread of the workgroup; returns next_label (uniform).
+template <typename Real>
+__device__ int condense_clusters(const TreeView<Real>& t, int n, long long mcs, ClusterSlot<Real>* slots,
+                                 int16_t* node_slot, const int16_t* pruned, int* counters) {
+  __shared__ int wave_sums[16];
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int m = n - 1;  // internal nodes n+i, i < m; the root is i = m-1
+  const int root = m - 1;
+  int16_t* par = t.uf;
+  int16_t* aux = t.comp;   // jump pointers, then split ranks, then path offsets
+  int16_t* dep = t.edge_v;
+  int16_t* pre = t.pparent;  // preorder, then pruned counts in path order
+  int16_t* hd = reinterpret_cast<int16_t*>(t.birth);
+  int16_t* lab = hd + n;
+  Real* pw = t.stab;  // lambda in path order
+  int16_t* pc = t.pparent;
+  constexpr int PER = 4;
+  for (int i = tid; i < m; i += T) {
+    const int a = t.left[i], c = t.right[i];
+    if (a >= n) par[a - n] = (int16_t)i;
+    if (c >= n) par[c - n] = (int16_t)i;
+  }
+  if (tid == 0) par[root] = -1;
+  __syncthreads();
+  // 2. depth and preorder.
+  bool kept[PER];
+  int vd[PER], vp[PER], va[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * T;
+    kept[r] = i < m && (i == root || t.sizes[i] >= mcs);
+    vd[r] = 0;
+    vp[r] = 0;
+    va[r] = -1;
+    if (kept[r] && i != root) {
+      const int p = par[i];
+      const int a = t.left[p];
+      vd[r] = 1;
+      vp[r] = a == n + i ? 1 : 2 * (a < n ? 1 : t.sizes[a - n]);
+      va[r] = p;
+    }
+    if (kept[r]) {
+      dep[i] = (int16_t)vd[r];
+      pre[i] = (int16_t)vp[r];
+      aux[i] = (int16_t)va[r];
+    }
+  }
+  __syncthreads();
+  for (;;) {
+    int nd[PER], np[PER], na[PER];
+    bool more = false;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      nd[r] = vd[r];
+      np[r] = vp[r];
+      na[r] = va[r];
+      if (kept[r] && va[r] >= 0) {
+        const int a = va[r];
+        nd[r] += dep[a];
+        np[r] += pre[a];
+        na[r] = aux[a];
+        more |= na[r] >= 0;
       }
-      if (!__ballot(active)) break;
-      bool split = false;
-      int si = 0, sa = 0;
-      Real lambda = 0;
-      if (active) {
-        if (x < n || x > 2 * n - 2) {  // only reachable for a malformed tree
-          slots[s].stab = stab;
-          slots[s].key = -1;
-          active = false;
-        } else {
-          const int i = x - n;
-          const EdgeKey<Real> info = t.keys[i];
-          const int count = pruned[i];
-          node_slot[i] = (int16_t)s;
-          lambda = info.w;
-          if ((info.idx >> 27) & 1) {
-            const int a = t.left[i], c = t.right[i];
-            sa = a < n ? 1 : t.sizes[a - n];
-            const int sc = c < n ? 1 : t.sizes[c - n];
-            stab += (lambda - b) * Real(sa);
-            stab += (lambda - b) * Real(sc);
-            slots[s].stab = stab;
-            slots[s].key = (d << 13) | p;
-            split = true;
-            si = i;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int i = tid + r * T;
+      if (kept[r] && va[r] >= 0) {
+        vd[r] = nd[r];
+        vp[r] = np[r];
+        va[r] = na[r];
+        dep[i] = (int16_t)nd[r];
+        pre[i] = (int16_t)np[r];
+        aux[i] = (int16_t)na[r];
+      }
+    }
+    if (!__syncthreads_or(more)) break;
+  }
+  // 3. first node of each kept node's cluster.
+  bool first[PER], split[PER];
+  int vh[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * T;
+    first[r] = false;
+    split[r] = false;
+    vh[r] = -1;
+    if (kept[r]) {
+      const int p = par[i];
+      first[r] = p < 0 || ((t.keys[p].idx >> 27) & 1);
+      split[r] = (t.keys[i].idx >> 27) & 1;
+      vh[r] = first[r] ? i : p;
+      hd[i] = (int16_t)vh[r];
+    }
+  }
+  __syncthreads();
+  for (;;) {
+    int nh[PER];
+    bool more = false;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      nh[r] = vh[r];
+      if (kept[r] && !first[r]) {
+        nh[r] = hd[vh[r]];
+        more |= nh[r] != vh[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int i = tid + r * T;
+      if (kept[r] && !first[r] && nh[r] != vh[r]) {
+        vh[r] = nh[r];
+        hd[i] = (int16_t)nh[r];
+      }
+    }
+    if (!__syncthreads_or(more)) break;
+  }
+  // 4. split ranks by (depth, preorder), cluster ids.  The split keys are
+  // compacted into the path-lambda storage (free until step 5).
+  int32_t* keys32 = reinterpret_cast<int32_t*>(pw);
+  if (tid == 0) counters[3] = 0;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; ++r)
+    if (split[r]) keys32[atomicAdd(&counters[3], 1)] = (vd[r] << 13) | vp[r];
+  __syncthreads();
+  const int splits = counters[3];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    if (split[r]) {
+      const int k = (vd[r] << 13) | vp[r];
+      int rank = 0;
+      for (int q = 0; q < splits; ++q) rank += keys32[q] < k ? 1 : 0;
+      aux[tid + r * T] = (int16_t)rank;
+    }
+  }
+  __syncthreads();
+  int vl[PER];  // cluster id - n of each first node
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * T;
+    vl[r] = -1;
+    if (first[r]) {
+      const int p = par[i];
+      vl[r] = p < 0 ? 0 : 1 + 2 * aux[p] + (t.left[p] == n + i ? 0 : 1);
+      lab[i] = (int16_t)vl[r];
+    }
+  }
+  __syncthreads();
+  const int count = 1 + 2 * splits;  // clusters, root included
+  // Path lengths: the last node of each path (a split, or no child kept).
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * T;
+    if (kept[r]) {
+      const int L = lab[vh[r]];
+      node_slot[i] = (int16_t)L;
+      if (split[r] || (t.keys[i].idx & 0x1fff) == 0) aux[L] = (int16_t)(vd[r] - dep[vh[r]] + 1);
+    }
+  }
+  // Per cluster: birth and parent cluster from its first node.
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    if (first[r]) {
+      const int i = tid + r * T;
+      const int p = par[i];
+      ClusterSlot<Real>& cs = slots[vl[r]];
+      cs.birth = p < 0 ? Real(0) : t.keys[p].w;
+      cs.parent = p < 0 ? -1 : n + lab[hd[p]];
+      cs.split_l = cs.split_r = 0;
+    }
+  }
+  __syncthreads();
+  block_exclusive_scan(aux, count, wave_sums);
+  // 5. path arrays.
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * T;
+    if (kept[r]) {
+      const int L = node_slot[i];
+      const int q = aux[L] + vd[r] - dep[vh[r]];
+      pw[q] = t.keys[i].w;
+      pc[q] = split[r] ? (int16_t)-1 : pruned[i];
+      if (split[r]) {
+        const int a = t.left[i], c = t.right[i];
+        slots[L].split_l = a < n ? 1 : t.sizes[a - n];
+        slots[L].split_r = c < n ? 1 : t.sizes[c - n];
+      }
+    }
+  }
+  __syncthreads();
+  Real cb[PER], cstab[PER];
+  int cpar[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int L = tid + r * T;
+    cb[r] = 0;
+    cstab[r] = 0;
+    cpar[r] = -1;
+    if (L < count) {
+      const ClusterSlot<Real> cs = slots[L];
+      const int q0 = aux[L], q1 = aux[L + 1];
+      const Real b = cs.birth;
+      Real stab = 0;
+      for (int q = q0; q < q1; q += 8) {
+        Real lw[8];
+        int lc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          lw[u] = q + u < q1 ? pw[q + u] : Real(0);
+          lc[u] = q + u < q1 ? pc[q + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const Real step = lw[u] - b;
+          if (lc[u] < 0) {
+            stab += step * Real(cs.split_l);
+            stab += step * Real(cs.split_r);
           } else {
-            const Real step = lambda - b;
-            for (int q = 0; q < count; ++q) stab += step;
-            const int next = info.idx & 0x1fff;
-            if (!next) {
-              slots[s].stab = stab;
-              slots[s].key = -1;
-              active = false;
-            } else {
-              p += (info.idx >> 13) & 0x3fff;
-              ++d;
-              x = next;
-            }
+            for (int k = 0; k < lc[u]; ++k) stab += step;
           }
         }
       }
-      const uint64_t sm = __ballot(split);
-      if (sm) {
-        // Two slots per split in lane order; a split past the slot capacity
-        // (n) creates no children, as in a level-by-level walk.
-        const int s1 = nalloc + 2 * (int)lanes_below(sm);
-        const bool room = split && s1 + 1 < n;
-        const uint64_t pm = __ballot(room);
-        if (room) {
-          const int a = t.left[si], c = t.right[si];
-          ClusterSlot<Real> ca, cc;
-          ca.birth = cc.birth = lambda;
-          ca.stab = cc.stab = 0;
-          ca.parent = cc.parent = s;
-          ca.key = cc.key = -1;
-          ca.depth = cc.depth = d + 1;
-          ca.label = cc.label = -1;
-          ca.root = a;
-          ca.pre = p + 1;
-          cc.root = c;
-          cc.pre = p + 2 * sa;
-          slots[s1] = ca;
-          slots[s1 + 1] = cc;
-          const int q = qtail + (int)lanes_below(pm);
-          queue[2 * q] = si | ((s1 + 1) << 13);
-          queue[2 * q + 1] = d | (p << 13);
-          s = s1;
-          x = a;
-          ++d;
-          ++p;
-          b = lambda;
-          stab = 0;
-        } else if (split) {
-          active = false;
-        }
-        nalloc += 2 * (int)__popcll(sm);
-        qtail += (int)__popcll(pm);
-      }
+      cb[r] = b;
+      cstab[r] = stab;
+      cpar[r] = cs.parent;
     }
-    if (lane == 0) counters[2] = nalloc;
   }
   __syncthreads();
-  const int nslots = min(counters[2], n);
-  // Rank the splits by (depth, preorder): stage the keys in LDS, count.
-  for (int s = tid; s < nslots; s += T) split_keys[s] = slots[s].key;
-  __syncthreads();
-  for (int s = tid; s < nslots; s += T) {
-    const int k = split_keys[s];
-    if (k < 0) continue;
-    int r = 0;
-    for (int q = 0; q < nslots; ++q) {
-      const int kq = split_keys[q];
-      r += (kq >= 0 && kq < k) ? 1 : 0;
+  // Final cluster arrays, indexed by cluster id - n (the scratch is dead).
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int L = tid + r * T;
+    if (L < count) {
+      t.birth[L] = cb[r];
+      t.stab[L] = cstab[r];
+      t.parent[L] = (int16_t)cpar[r];
+      t.selected[L] = (uint8_t)(L == 0 ? 0 : 1);
     }
-    slots[s].label = r;  // rank for now
   }
   __syncthreads();
-  int splits = 0;
-  for (int s = tid; s < nslots; s += T) splits += split_keys[s] >= 0 ? 1 : 0;
-  if (tid == 0) counters[3] = 0;
-  __syncthreads();
-  if (splits) atomicAdd(&counters[3], splits);
-  // Labels of children: n + 1 + 2 * rank(parent) + (second child).
-  for (int s = tid; s < nslots; s += T) {
-    if (s == 0) continue;
-    const int par = slots[s].parent;
-    split_keys[s] = n + 1 + 2 * slots[par].label + ((s - 1) & 1);
-  }
-  if (tid == 0) split_keys[0] = n;
-  __syncthreads();
-  // Final cluster arrays, indexed by label - n.
-  for (int s = tid; s < nslots; s += T) {
-    const int lab = split_keys[s];
-    const int c = lab - n;
-    t.birth[c] = slots[s].birth;
-    t.stab[c] = slots[s].stab;
-    t.parent[c] = (int16_t)(s == 0 ? -1 : split_keys[slots[s].parent]);
-    t.selected[c] = (uint8_t)(s == 0 ? 0 : 1);
-    slots[s].label = lab;
-  }
-  __syncthreads();
-  return n + 1 + 2 * counters[3];
+  return n + count;
 }
 
-// Point parents: each point climbs to its first ancestor visited by a walk
-// (the node that pruned it) and takes that node's cluster.
+// Point parents: each point climbs to its first kept ancestor (the node that
+// pruned it) and takes that node's cluster.
 template <typename Real>
 __device__ void condense_points(const TreeView<Real>& t, int n, const int16_t* parent_of,
-                                const int16_t* node_slot, const ClusterSlot<Real>* slots) {
+                                const int16_t* node_slot) {
   for (int leaf = threadIdx.x; leaf < n; leaf += blockDim.x) {
     int x = parent_of[leaf], guard = 0;
     while (x >= n && node_slot[x - n] < 0 && guard++ < n) x = parent_of[x];
     const int s = x >= n ? node_slot[x - n] : -1;
-    t.pparent[leaf] = (int16_t)(s >= 0 ? slots[s].label : -1);
+    t.pparent[leaf] = (int16_t)(s >= 0 ? n + s : -1);
   }
 }
 
