@@ -488,6 +488,9 @@ __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, 
     }
     int ol = 0, orr = 0, os = 0;
     const int cnt = min(64, n - 1 - i0);
+    // Wait for the root and size loads here: a wait inside the loop below
+    // would also wait, every edge, for the previous edge's LDS stores.
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
     for (int k = 0; k < cnt; ++k) {
       const uint32_t roots = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)ra | ((uint32_t)rc << 16)), k);
       const uint32_t sizes = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)sa | ((uint32_t)sc << 16)), k);
@@ -496,15 +499,16 @@ __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, 
       const int ns = ksa + ksc;
       const int win = ksa < ksc ? krc : kra, los = ksa < ksc ? kra : krc;
       const int node = n + i0 + k;
-      if (lane == k) {
-        // This lane's own node ids are the edge's children.
-        ol = ca;
-        orr = cc;
-        os = ns;
-        uf[los] = (int16_t)win;
-        rsize[win] = (int16_t)ns;
-        t.comp[win] = (int16_t)node;
-      }
+      // Lane k's own node ids are the edge's children.
+      const bool mine = lane == k;
+      ol = mine ? ca : ol;
+      orr = mine ? cc : orr;
+      os = mine ? ns : os;
+      // Every lane stores the same values to the same addresses: no branch
+      // around a one-lane store.
+      uf[los] = (int16_t)win;
+      rsize[win] = (int16_t)ns;
+      t.comp[win] = (int16_t)node;
       if (ra == los) ra = win;
       if (rc == los) rc = win;
       if (ra == win) {
