@@ -204,52 +204,89 @@ __device__ int wave_partition(EdgeKey<Real>* keys, int lo, int hi, int16_t* lpos
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   const Real pivot = lane0_value(piv);
-  // Positions >= pivot in [lo+1, hi-1], ascending.
-  int n_ge = 0;
-  for (int base = lo + 1; base <= hi - 1; base += 64) {
+  // One ascending pass over [lo, hi-1]: positions >= pivot in [lo+1, hi-1]
+  // (L_1 < L_2 < ...) and positions <= pivot in [lo, hi-2], also ascending
+  // (R_k, the k-th from the top, is entry n_le - k).
+  int n_ge = 0, n_le = 0;
+  for (int base = lo; base <= hi - 1; base += 64) {
     const int p = base + lane;
-    const bool ge = p <= hi - 1 && !(keys[p].w < pivot);
-    const uint64_t m = __ballot(ge);
-    if (ge) lpos[lo + n_ge + (int)lanes_below(m)] = (int16_t)p;
-    n_ge += __popcll(m);
-  }
-  // Positions <= pivot in [lo, hi-2], descending.
-  int n_le = 0;
-  for (int top = hi - 2; top >= lo; top -= 64) {
-    const int p = top - lane;
-    const bool le = p >= lo && !(pivot < keys[p].w);
-    const uint64_t m = __ballot(le);
-    if (le) rpos[lo + n_le + (int)lanes_below(m)] = (int16_t)p;
-    n_le += __popcll(m);
+    const bool in = p <= hi - 1;
+    const Real w = in ? keys[p].w : Real(0);
+    const bool ge = in && p >= lo + 1 && !(w < pivot);
+    const bool le = in && p <= hi - 2 && !(pivot < w);
+    const uint64_t mg = __ballot(ge), ml = __ballot(le);
+    if (ge) lpos[lo + n_ge + (int)lanes_below(mg)] = (int16_t)p;
+    if (le) rpos[lo + n_le + (int)lanes_below(ml)] = (int16_t)p;
+    n_ge += __popcll(mg);
+    n_le += __popcll(ml);
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+  const int16_t* rtop = rpos + lo + n_le - 1;  // rtop[-(k-1)] = R_k
   // K = number of leading k with L_k < R_k (a prefix property).
   const int mlen = min(n_ge, n_le);
   int swaps = 0;
   for (int base = 0; base < mlen; base += 64) {
     const int k = base + lane;
     const bool valid = k < mlen;
-    const bool c = valid && lpos[lo + k] < rpos[lo + k];
+    const bool c = valid && lpos[lo + k] < rtop[-k];
     swaps += __popcll(__ballot(c));
     if (__ballot(valid && !c) != 0) break;  // the first false ends the prefix
   }
   for (int base = 0; base < swaps; base += 64) {
     const int k = base + lane;
     if (k < swaps) {
-      const int a = lpos[lo + k], b = rpos[lo + k];
+      const int a = lpos[lo + k], b = rtop[-k];
       const EdgeKey<Real> ka = keys[a], kb = keys[b];
       keys[a] = kb;
       keys[b] = ka;
     }
   }
-  const int i = swaps == 0 ? lpos[lo] : min((int)lpos[lo + swaps], (int)rpos[lo + swaps - 1]);
+  const int i = swaps == 0 ? lpos[lo] : min((int)lpos[lo + swaps], (int)rtop[-(swaps - 1)]);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) key_swap(keys, i, hi - 1);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   return i;
+}
+
+// NumPy's insertion sort of keys[lo..hi] (hi - lo <= 15) in registers.
+// Insertion sort with a strict comparison is stable, so its result is the
+// stable order of the span; adjacent compare-exchanges that swap only on a
+// strict inequality (odd-even transposition, as many rounds as slots) give
+// exactly that order.  Slots past the span never move.
+template <typename Real>
+__device__ void small_sort(EdgeKey<Real>* keys, int lo, int hi) {
+  constexpr int S = kSmallSort + 1;
+  const int len = hi - lo + 1;
+  Real w[S];
+  int32_t id[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    w[q] = q < len ? keys[lo + q].w : Real(0);
+    id[q] = q < len ? keys[lo + q].idx : 0;
+  }
+#pragma unroll
+  for (int round = 0; round < S; ++round) {
+#pragma unroll
+    for (int q = round & 1; q + 1 < S; q += 2) {
+      const bool sw = q + 1 < len && w[q + 1] < w[q];
+      const Real a = w[q], b = w[q + 1];
+      const int32_t ia = id[q], ib = id[q + 1];
+      w[q] = sw ? b : a;
+      w[q + 1] = sw ? a : b;
+      id[q] = sw ? ib : ia;
+      id[q + 1] = sw ? ia : ib;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    if (q < len) {
+      keys[lo + q].w = w[q];
+      keys[lo + q].idx = id[q];
+    }
+  }
 }
 
 // Task lists of the parallel introsort (int16 entries).  Spans pushed in one
@@ -337,18 +374,7 @@ __device__ void parallel_introsort(EdgeKey<Real>* keys, int m, int depth_overrid
     __syncthreads();
   }
   const int nsmall = counters[2], nheap = counters[3];
-  for (int s = tid; s < nsmall; s += blockDim.x) {
-    const int lo = L.small[s], hi = L.small[L.cap_small + s];
-    for (int i = lo + 1; i <= hi; ++i) {
-      EdgeKey<Real> v = keys[i];
-      int j = i;
-      while (j > lo && v.w < keys[j - 1].w) {
-        keys[j] = keys[j - 1];
-        --j;
-      }
-      keys[j] = v;
-    }
-  }
+  for (int s = tid; s < nsmall; s += blockDim.x) small_sort(keys, L.small[s], L.small[L.cap_small + s]);
   for (int h = tid; h < nheap; h += blockDim.x) heap_sort(keys, (int)L.heap[h], (int)L.heap[L.cap + h]);
   __syncthreads();
 }
