@@ -412,13 +412,13 @@ __global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
     // parent / point_parent / birth are free until condensation.
     int16_t* su = t.parent;
     int16_t* sv = t.pparent;
-    int16_t* rsize = reinterpret_cast<int16_t*>(t.birth);
+    uint32_t* rec = reinterpret_cast<uint32_t*>(t.birth);
     clk[1] = wall_clock64();
-    linkage_gather(t, n, su, sv, rsize);
+    linkage_gather(t, n, su, sv, rec);
     __syncthreads();
     if (tid < 64) {
       clk[2] = wall_clock64();
-      linkage_wave(t, n, su, sv, rsize);
+      linkage_wave(t, n, su, sv, rec);
       clk[3] = wall_clock64();
     }
   }

@@ -426,14 +426,13 @@ __device__ __forceinline__ TreeView<Real> tree_view(unsigned char* base, int n) 
 }
 
 // Kruskal over the sorted keys: node n+i joins the newest nodes containing
-// edge_u and edge_v of the i-th lightest edge.  Union by size with path
-// halving; representatives stay leaves so comp[] maps them to their newest
-// node and rsize[] holds their component size.  left/right/sizes are indexed
-// by node - n.  su/sv hold the endpoints already gathered in sorted order
-// (linkage_gather), so the only dependent loads per edge are the two finds,
-// which are advanced together.
+// edge_u and edge_v of the i-th lightest edge.  Union by size;
+// representatives stay leaves, and rec[] holds each representative's newest
+// node id | component size << 16 (one 32-bit LDS word: ids < 2^13, sizes <=
+// 4096).  left/right/sizes are indexed by node - n.  su/sv hold the
+// endpoints already gathered in sorted order (linkage_gather).
 template <typename Real>
-__device__ void linkage_gather(const TreeView<Real>& t, int n, int16_t* su, int16_t* sv, int16_t* rsize) {
+__device__ void linkage_gather(const TreeView<Real>& t, int n, int16_t* su, int16_t* sv, uint32_t* rec) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     if (i < n - 1) {
       const int e = t.keys[i].idx;
@@ -441,115 +440,70 @@ __device__ void linkage_gather(const TreeView<Real>& t, int n, int16_t* su, int1
       sv[i] = t.edge_v[e];
     }
     t.uf[i] = (int16_t)i;
-    t.comp[i] = (int16_t)i;
-    rsize[i] = 1;
+    rec[i] = (uint32_t)i | (1u << 16);
   }
 }
 
-// Run by one lane; see linkage_gather for the inputs.
-template <typename Real>
-__device__ void linkage(const TreeView<Real>& t, int n, const int16_t* su, const int16_t* sv, int16_t* rsize) {
-  int16_t* uf = t.uf;
-  for (int i = 0; i < n - 1; ++i) {
-    int a = su[i], c = sv[i];
-    for (;;) {
-      const int pa = uf[a], pc = uf[c];
-      if (pa == a && pc == c) break;
-      if (pa != a) {
-        const int g = uf[pa];
-        uf[a] = (int16_t)g;
-        a = g;
-      }
-      if (pc != c) {
-        const int g = uf[pc];
-        uf[c] = (int16_t)g;
-        c = g;
-      }
-    }
-    const int ca = t.comp[a], cc = t.comp[c];
-    const int sa = rsize[a], sc = rsize[c];
-    t.left[i] = (int16_t)ca;
-    t.right[i] = (int16_t)cc;
-    t.sizes[i] = (int16_t)(sa + sc);
-    if (sa < sc) {
-      uf[a] = (int16_t)c;
-      t.comp[c] = (int16_t)(n + i);
-      rsize[c] = (int16_t)(sa + sc);
-    } else {
-      uf[c] = (int16_t)a;
-      t.comp[a] = (int16_t)(n + i);
-      rsize[a] = (int16_t)(sa + sc);
-    }
-  }
-}
-
-// The same linkage run by one wave, 64 edges at a time.  Each lane finds
-// the roots of its edge's endpoints in the union-find state at the start of
-// the batch (with their sizes and current node ids); the unions are then
-// applied in edge order, each broadcast from its lane so that later lanes
-// replace a root that just lost by the winner and refresh the winner's size
-// and node id.  Every edge therefore sees exactly the roots, sizes and node
-// ids the sequential algorithm sees, and the unions (by size, ties to the
-// first endpoint's root) are the same.  Depth stays <= log2(n) without path
-// compression because unions are by size.
+// The linkage run by one wave, 64 edges at a time.  Each lane finds the
+// roots of its edge's endpoints in the union-find state at the start of the
+// batch (with their records); the unions are then applied in edge order,
+// each broadcast from its lane so that later lanes replace a root that just
+// lost by the winner and take the winner's new record.  Every edge therefore
+// sees exactly the roots, sizes and node ids the sequential algorithm sees,
+// and the unions (by size, ties to the first endpoint's root) are the same.
+// Depth stays <= log2(n) without path compression because unions are by
+// size.  Per edge: three lane reads, two compares and two selects per root.
 template <typename Real>
 __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, const int16_t* sv,
-                             int16_t* rsize) {
+                             uint32_t* rec) {
   const int lane = threadIdx.x & 63;
   int16_t* uf = t.uf;
   for (int i0 = 0; i0 < n - 1; i0 += 64) {
     const int i = i0 + lane;
     const bool valid = i < n - 1;
-    int ra = 0, rc = 0, ca = 0, cc = 0, sa = 0, sc = 0;
+    int ra = 0, rc = 0;
+    uint32_t ka = 0, kc = 0;  // records of the two roots
     if (valid) {
       int a = su[i], c = sv[i];
       for (int p = uf[a]; p != a; p = uf[a]) a = p;
       for (int p = uf[c]; p != c; p = uf[c]) c = p;
       ra = a;
       rc = c;
-      ca = t.comp[a];
-      cc = t.comp[c];
-      sa = rsize[a];
-      sc = rsize[c];
+      ka = rec[a];
+      kc = rec[c];
     }
-    int ol = 0, orr = 0, os = 0;
+    uint32_t oa = 0, oc = 0;  // this lane's records at its own edge
     const int cnt = min(64, n - 1 - i0);
-    // Wait for the root and size loads here: a wait inside the loop below
+    // Wait for the root and record loads here: a wait inside the loop below
     // would also wait, every edge, for the previous edge's LDS stores.
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
     for (int k = 0; k < cnt; ++k) {
       const uint32_t roots = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)ra | ((uint32_t)rc << 16)), k);
-      const uint32_t sizes = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)sa | ((uint32_t)sc << 16)), k);
+      const uint32_t ea = (uint32_t)__builtin_amdgcn_readlane((int)ka, k);
+      const uint32_t ec = (uint32_t)__builtin_amdgcn_readlane((int)kc, k);
       const int kra = (int)(roots & 0xFFFFu), krc = (int)(roots >> 16);
-      const int ksa = (int)(sizes & 0xFFFFu), ksc = (int)(sizes >> 16);
-      const int ns = ksa + ksc;
-      const int win = ksa < ksc ? krc : kra, los = ksa < ksc ? kra : krc;
-      const int node = n + i0 + k;
-      // Lane k's own node ids are the edge's children.
+      const uint32_t sa = ea >> 16, sc = ec >> 16;
+      const int win = sa < sc ? krc : kra, los = sa < sc ? kra : krc;
+      const uint32_t merged = (uint32_t)(n + i0 + k) | ((sa + sc) << 16);
+      // Lane k's own records hold the edge's children and sizes.
       const bool mine = lane == k;
-      ol = mine ? ca : ol;
-      orr = mine ? cc : orr;
-      os = mine ? ns : os;
+      oa = mine ? ka : oa;
+      oc = mine ? kc : oc;
       // Every lane stores the same values to the same addresses: no branch
       // around a one-lane store.
       uf[los] = (int16_t)win;
-      rsize[win] = (int16_t)ns;
-      t.comp[win] = (int16_t)node;
-      if (ra == los) ra = win;
-      if (rc == los) rc = win;
-      if (ra == win) {
-        ca = node;
-        sa = ns;
-      }
-      if (rc == win) {
-        cc = node;
-        sc = ns;
-      }
+      rec[win] = merged;
+      const bool ma = ra == los || ra == win;
+      const bool mc = rc == los || rc == win;
+      ra = ma ? win : ra;
+      ka = ma ? merged : ka;
+      rc = mc ? win : rc;
+      kc = mc ? merged : kc;
     }
     if (valid) {
-      t.left[i] = (int16_t)ol;
-      t.right[i] = (int16_t)orr;
-      t.sizes[i] = (int16_t)os;
+      t.left[i] = (int16_t)(oa & 0xFFFFu);
+      t.right[i] = (int16_t)(oc & 0xFFFFu);
+      t.sizes[i] = (int16_t)((oa >> 16) + (oc >> 16));
     }
   }
 }
