@@ -224,8 +224,7 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   // N=1024 without core distances (fewer waves to synchronise per step, two
   // slots of ILP), 1024 threads with core distances and at most 16 fp32
   // coordinates per point (1.5 % faster at N=1024, D=16); larger windows
-  // grow the block up to 1024 threads when registers would not hold the
-  // coordinates of more slots.
+  // use 1024 threads (four slots per thread past 2048 points).
   const int dk = pad_dims(d);
   P.dk = dk;
   const int want_vpt = (flags >> 8) & 15;
@@ -235,6 +234,9 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   if (!vpt) {
     vpt = n <= 512 ? 1 : n <= 1024 ? 2 : 4;
     if (vpt == 2 && P.core && dk > 0 && (int64_t)dk * rs <= 64) vpt = 1;
+    // Points path up to 2048: two slots per thread and 1024 threads beat
+    // four slots and 512 threads (20 % at N=2048, D=16).
+    if (vpt == 4 && n <= 2048 && dk > 0 && (int64_t)2 * dk * rs <= 64 * 4) vpt = 2;
     while (vpt > 1 && (int64_t)vpt * dk * rs > 64 * 4 && (int64_t)(vpt / 2) * 1024 >= n) vpt /= 2;
   }
   while ((int64_t)vpt * 1024 < n) vpt *= 2;
