@@ -2,9 +2,12 @@
 // union-find linkage, breadth-first condensation, excess-of-mass selection,
 // epsilon merge and label resolution.
 //
-// These are sequential by definition (each step depends on the previous
-// one); one lane runs them out of the window's arena (LDS when it fits).
-// Semantics follow the reference stage by stage:
+// One workgroup per window runs them out of the window's arena (LDS when it
+// fits), each stage in the parallel form that gives the reference's exact
+// result: partitions by one wave each, linkage broadcast through one wave,
+// condensation by pointer jumping with sequential per-cluster sums,
+// selection level by level on one wave.  Semantics follow the reference
+// stage by stage:
 //   sort      reference hdbscan_gpu/_kernels.py:358-516
 //   linkage   reference hdbscan_gpu/_kernels.py:518-556
 //   condense  reference hdbscan_gpu/_kernels.py:558-643
@@ -542,13 +545,9 @@ __device__ void condense_prepare(const TreeView<Real>& t, int n, long long mcs, 
 // indexed by cluster id - n).
 template <typename Real> struct ClusterSlot {
   Real birth;        // lambda at which the cluster appears (0 for the root)
-  Real stab;         // unused (the stability stays in registers)
   int32_t parent;    // parent cluster id (-1 for the root)
-  int32_t start;     // first entry of the cluster's path in the path arrays
-  int32_t len;       // nodes on the path
   int32_t split_l;   // sizes of the children of the path's last node when it
   int32_t split_r;   //   splits (0 otherwise)
-  int32_t pad;
 };
 
 // Exclusive prefix sum of v[0..count-1] in place (count <= 4 * blockDim.x),

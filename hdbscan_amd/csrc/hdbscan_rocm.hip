@@ -1,16 +1,20 @@
 // C ABI of the gfx950 batched HDBSCAN hot path (see include/hdbscan_rocm.h).
 //
 // Pipeline per call, all on the caller's stream:
-//   [core]  core_topk_kernel (min_samples 3..64, from points) or
-//           distance_kernel + core_select_kernel (min_samples > 64)
+//   [core]  gram_prep_kernel + core_gram_kernel + core_exact_kernel (the
+//           certified MFMA Gram filter: fp32, min_samples 3..16, points
+//           path), core_topk_kernel (exact top-k from points), or
+//           core_select_list_kernel / core_select_kernel (selection from a
+//           materialised matrix)
 //   [prim]  prim_kernel: points path (distances formed on the fly from
-//           register/LDS-resident coordinates) or matrix path (distance_kernel
-//           first, Prim reads rows)
+//           register/LDS-resident coordinates), or distance_kernel and then
+//           prim_spec_kernel reading matrix rows (two per round)
 //   [tree]  tree_kernel: sort, linkage, condensation, selection, labels
 // Workspace (caller-owned device memory, hdb_workspace_bytes):
 //   edge_u[B][n] i32 | edge_v[B][n] i32 | edge_w[B][n] Real | wflag[B] i32 |
-//   core[B][n] Real | tree arena[B][stride] | cluster slots[B][n] |
-//   distances[B][n][n] Real
+//   core[B][n] Real | tree arena[B][stride] | cluster records[B][n] |
+//   distances[B][n][n] Real (matrix path) | Gram candidates, operands and
+//   norm terms (filter) | Prim repacking records (points path)
 #include <hip/hip_runtime.h>
 
 #include <mutex>
