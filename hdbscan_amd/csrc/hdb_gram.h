@@ -445,14 +445,40 @@ __global__ __launch_bounds__(256) void core_exact_kernel(const float* x, const f
 #pragma unroll
     for (int q = 0; q < KMAX; ++q) lst[q] = INFINITY;
     if (m <= CAP) {
-      for (int p = 0; p < m; ++p) {
-        const int j = cand[row * CAP + p];
+      // Four candidates at a time, their gathers in flight together (the
+      // K smallest values do not depend on the order they are offered in).
+      // The list's slots are 64-byte aligned, so four entries are one
+      // 8-byte load.
+      const uint16_t* cr = cand + row * CAP;
+      int p = 0;
+      for (; p + 4 <= m; p += 4) {
+        const uint2 jj = *reinterpret_cast<const uint2*>(cr + p);
+        const int j[4] = {(int)(jj.x & 0xFFFFu), (int)(jj.x >> 16), (int)(jj.y & 0xFFFFu), (int)(jj.y >> 16)};
+        float t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] = di ? di[j[u]] : exact_sq(xi, xb + (size_t)j[u] * d, d);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) list_offer<KMAX>(lst, t[u]);
+      }
+      for (; p < m; ++p) {
+        const int j = cr[p];
         // The matrix holds sqrt(s); squares of its values order like s.
         const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
         list_offer<KMAX>(lst, t);
       }
     } else {
-      for (int j = 0; j < n; ++j) {
+      int j = 0;
+      for (; j + 4 <= n; j += 4) {
+        float t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] = di ? di[j + u] : exact_sq(xi, xb + (size_t)(j + u) * d, d);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          bad |= !isfinite(t[u]);
+          list_offer<KMAX>(lst, t[u]);
+        }
+      }
+      for (; j < n; ++j) {
         const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
         bad |= !isfinite(t);
         list_offer<KMAX>(lst, t);
