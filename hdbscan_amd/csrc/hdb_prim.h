@@ -202,15 +202,21 @@ __device__ __forceinline__ uint32_t wave_umin32(uint32_t v) {
   return read_lane(v, 63);
 }
 
-// The fp32 candidate key in two 32-bit passes: the smallest weight pattern,
-// then the smallest vertex among the lanes holding it.  That is the
-// lexicographic minimum of (weight, vertex), the 64-bit key minimum, in 12
-// single-instruction levels instead of six levels of 64-bit compares and
-// selects.
+// The fp32 candidate key in two 32-bit steps: the smallest weight pattern
+// (six single-instruction levels), then the smallest vertex among the lanes
+// holding it -- the lexicographic minimum of (weight, vertex), i.e. the
+// 64-bit key minimum.  One lane holds that weight almost always (a wave's
+// minimum is tied in about 1 step in 300 at N=1024, D=16): its vertex is
+// read from it directly; ties take a second six-level pass over vertices.
 __device__ __forceinline__ Cand<float> wave_min(Cand<float> c) {
   const uint32_t w = (uint32_t)(c.k >> 32);
   const uint32_t wm = wave_umin32(w);
-  const uint32_t jm = wave_umin32(w == wm ? (uint32_t)c.k : 0xFFFFFFFFu);
+  const uint64_t tied = __builtin_amdgcn_ballot_w64(w == wm);
+  uint32_t jm;
+  if (__builtin_popcountll(tied) == 1)
+    jm = read_lane((uint32_t)c.k, (int)__builtin_ctzll(tied));
+  else
+    jm = wave_umin32(w == wm ? (uint32_t)c.k : 0xFFFFFFFFu);
   Cand<float> r;
   r.k = ((uint64_t)wm << 32) | jm;
   return r;
