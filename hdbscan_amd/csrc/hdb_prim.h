@@ -222,6 +222,39 @@ __device__ __forceinline__ Cand<float> wave_min(Cand<float> c) {
   return r;
 }
 
+// Wave-wide unsigned 64-bit minimum, the same lane pattern as wave_min.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  return ((uint64_t)dpp_rows<CTRL, ROWS>((uint32_t)(v >> 32)) << 32) | dpp_rows<CTRL, ROWS>((uint32_t)v);
+}
+__device__ __forceinline__ uint64_t read_lane64(uint64_t v, int lane) {
+  return ((uint64_t)read_lane((uint32_t)(v >> 32), lane) << 32) | read_lane((uint32_t)v, lane);
+}
+__device__ __forceinline__ uint64_t wave_umin64(uint64_t v) {
+  v = min(v, dpp_u64<kDppXor1, 0xF>(v));
+  v = min(v, dpp_u64<kDppXor2, 0xF>(v));
+  v = min(v, dpp_u64<kDppRor4, 0xF>(v));
+  v = min(v, dpp_u64<kDppRor8, 0xF>(v));
+  v = min(v, dpp_u64<kDppBcast15, 0xA>(v));
+  v = min(v, dpp_u64<kDppBcast31, 0xC>(v));
+  return read_lane64(v, 63);
+}
+
+// The fp64 candidate key in the same two steps as the fp32 one: the
+// smallest weight pattern (64-bit levels), then the vertex from the only
+// lane holding it, or a second pass over vertices when it is tied.
+__device__ __forceinline__ Cand<double> wave_min(Cand<double> c) {
+  const uint64_t wm = wave_umin64(c.w);
+  const uint64_t tied = __builtin_amdgcn_ballot_w64(c.w == wm);
+  Cand<double> r;
+  r.w = wm;
+  if (__builtin_popcountll(tied) == 1)
+    r.j = read_lane(c.j, (int)__builtin_ctzll(tied));
+  else
+    r.j = wave_umin32(c.w == wm ? c.j : 0xFFFFFFFFu);
+  return r;
+}
+
 // Minimum over lanes 0..15, returned wave-uniform (the other rows are not
 // read).  All 64 lanes must be active.
 template <typename Real>
@@ -236,6 +269,24 @@ __device__ __forceinline__ uint32_t row0_umin32(uint32_t v) {
   v = min(v, dpp_umin_src<kDppRor4, 0xF>(v));
   v = min(v, dpp_umin_src<kDppRor8, 0xF>(v));
   return read_lane(v, 0);
+}
+
+// fp64 keys over lanes 0..15, as the fp64 wave_min.
+__device__ __forceinline__ Cand<double> row0_min(Cand<double> c) {
+  uint64_t v = c.w;
+  v = min(v, dpp_u64<kDppXor1, 0xF>(v));
+  v = min(v, dpp_u64<kDppXor2, 0xF>(v));
+  v = min(v, dpp_u64<kDppRor4, 0xF>(v));
+  v = min(v, dpp_u64<kDppRor8, 0xF>(v));
+  const uint64_t wm = read_lane64(v, 0);
+  const uint64_t tied = __builtin_amdgcn_ballot_w64(c.w == wm) & 0xFFFFull;
+  Cand<double> r;
+  r.w = wm;
+  if (__builtin_popcountll(tied) == 1)
+    r.j = read_lane(c.j, (int)__builtin_ctzll(tied));
+  else
+    r.j = row0_umin32(c.w == wm ? c.j : 0xFFFFFFFFu);
+  return r;
 }
 
 // fp32 keys over lanes 0..15 in two 32-bit passes, as wave_min.
