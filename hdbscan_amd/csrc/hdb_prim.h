@@ -558,7 +558,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
     const Real ccur = unit ? Real(0) : cs[cur];
     const Real ccx = unit ? Real(-INFINITY) : ccur;
     const Real ncur = gram ? xn[cur] : Real(0);
-    C c = C::make(rmax, 0);
+    C c = C::none();
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
       // Only a weight strictly below best can change this vertex.  With core
@@ -617,9 +617,15 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
         }
       }
       // Visited slots hold C::none(), above every candidate.
-      c = cand_pick(key[s].less(c), key[s], c);
+      c = s == 0 ? key[0] : cand_pick(key[s].less(c), key[s], c);
     }
     c = wave_min(c);
+    {
+      // The start value (REAL_MAX, 0) of the reference's argmin, once per
+      // wave on its minimum instead of once per lane.
+      const C start = C::make(rmax, 0);
+      c = cand_pick(c.less(start), c, start);
+    }
     if constexpr (WSLOT) {
       // The wave's argmin vertex, if this lane owns it, publishes its
       // coordinates (a wave with every vertex visited publishes nothing).
