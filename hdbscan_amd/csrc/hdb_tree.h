@@ -546,8 +546,6 @@ __device__ void condense_prepare(const TreeView<Real>& t, int n, long long mcs, 
 template <typename Real> struct ClusterSlot {
   Real birth;        // lambda at which the cluster appears (0 for the root)
   int32_t parent;    // parent cluster id (-1 for the root)
-  int32_t split_l;   // sizes of the children of the path's last node when it
-  int32_t split_r;   //   splits (0 otherwise)
 };
 
 // Exclusive prefix sum of v[0..count-1] in place (count <= 4 * blockDim.x),
@@ -598,8 +596,8 @@ __device__ void block_exclusive_scan(int16_t* v, int count, int* wave_sums) {
 //   3. the first node of each kept node's cluster by pointer jumping;
 //   4. split ranks by counting over the compacted (depth, preorder) keys,
 //      cluster ids, node_slot[i] = cluster id - n of each kept node n+i;
-//   5. path arrays (lambda and pruned count per node, in path order) and
-//      each cluster's sequential stability sum.
+//   5. every path's additions laid out in path order (a scan of their
+//      counts places each node's), then each cluster's sequential sum.
 // Scratch: uf, comp, edge_v, point_parent, birth and stability (all dead or
 // not yet written).  Leaves birth, stability, parent and selected per cluster
 // (indexed by cluster id - n).  Needs n - 1 <= 4 * blockDim.x.  Called by
@@ -617,8 +615,7 @@ __device__ int condense_clusters(const TreeView<Real>& t, int n, long long mcs, 
   int16_t* pre = t.pparent;  // preorder, then pruned counts in path order
   int16_t* hd = reinterpret_cast<int16_t*>(t.birth);
   int16_t* lab = hd + n;
-  Real* pw = t.stab;  // lambda in path order
-  int16_t* pc = t.pparent;
+  int16_t* pc = t.pparent;  // additions per path position, then their offsets
   constexpr int PER = 4;
   for (int i = tid; i < m; i += T) {
     const int a = t.left[i], c = t.right[i];
@@ -724,7 +721,7 @@ __device__ int condense_clusters(const TreeView<Real>& t, int n, long long mcs, 
   }
   // 4. split ranks by (depth, preorder), cluster ids.  The split keys are
   // compacted into the path-lambda storage (free until step 5).
-  int32_t* keys32 = reinterpret_cast<int32_t*>(pw);
+  int32_t* keys32 = reinterpret_cast<int32_t*>(t.stab);
   if (tid == 0) counters[3] = 0;
   __syncthreads();
 #pragma unroll
@@ -774,24 +771,40 @@ __device__ int condense_clusters(const TreeView<Real>& t, int n, long long mcs, 
       ClusterSlot<Real>& cs = slots[vl[r]];
       cs.birth = p < 0 ? Real(0) : t.keys[p].w;
       cs.parent = p < 0 ? -1 : n + lab[hd[p]];
-      cs.split_l = cs.split_r = 0;
     }
   }
   __syncthreads();
   block_exclusive_scan(aux, count, wave_sums);
-  // 5. path arrays.
+  // 5. the additions of every path, in path order: per position the
+  // number of additions (the pruned points, or the two children of a
+  // split), an exclusive scan for each position's first one, then every
+  // kept node writes its own; one lane per cluster adds its range in order.
+  const int npos = aux[count];
+  int qpos[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = tid + r * T;
+    qpos[r] = 0;
+    if (kept[r]) {
+      qpos[r] = aux[node_slot[i]] + vd[r] - dep[vh[r]];
+      pc[qpos[r]] = split[r] ? (int16_t)2 : pruned[i];
+    }
+  }
+  __syncthreads();
+  block_exclusive_scan(pc, npos, wave_sums);
+  Real* ev = t.birth;  // birth and stability regions, contiguous: >= 2n Reals
 #pragma unroll
   for (int r = 0; r < PER; ++r) {
     const int i = tid + r * T;
     if (kept[r]) {
-      const int L = node_slot[i];
-      const int q = aux[L] + vd[r] - dep[vh[r]];
-      pw[q] = t.keys[i].w;
-      pc[q] = split[r] ? (int16_t)-1 : pruned[i];
+      const Real step = t.keys[i].w - slots[node_slot[i]].birth;
+      const int e0 = pc[qpos[r]];
       if (split[r]) {
         const int a = t.left[i], c = t.right[i];
-        slots[L].split_l = a < n ? 1 : t.sizes[a - n];
-        slots[L].split_r = c < n ? 1 : t.sizes[c - n];
+        ev[e0] = step * Real(a < n ? 1 : t.sizes[a - n]);
+        ev[e0 + 1] = step * Real(c < n ? 1 : t.sizes[c - n]);
+      } else {
+        for (int k = 0; k < pruned[i]; ++k) ev[e0 + k] = step;
       }
     }
   }
@@ -806,29 +819,18 @@ __device__ int condense_clusters(const TreeView<Real>& t, int n, long long mcs, 
     cpar[r] = -1;
     if (L < count) {
       const ClusterSlot<Real> cs = slots[L];
-      const int q0 = aux[L], q1 = aux[L + 1];
-      const Real b = cs.birth;
+      const int e1 = pc[aux[L + 1]];
+      int e = pc[aux[L]];
       Real stab = 0;
-      for (int q = q0; q < q1; q += 8) {
-        Real lw[8];
-        int lc[8];
+      for (; e + 8 <= e1; e += 8) {
+        Real v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          lw[u] = q + u < q1 ? pw[q + u] : Real(0);
-          lc[u] = q + u < q1 ? pc[q + u] : 0;
-        }
+        for (int u = 0; u < 8; ++u) v[u] = ev[e + u];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const Real step = lw[u] - b;
-          if (lc[u] < 0) {
-            stab += step * Real(cs.split_l);
-            stab += step * Real(cs.split_r);
-          } else {
-            for (int k = 0; k < lc[u]; ++k) stab += step;
-          }
-        }
+        for (int u = 0; u < 8; ++u) stab += v[u];
       }
-      cb[r] = b;
+      for (; e < e1; ++e) stab += ev[e];
+      cb[r] = cs.birth;
       cstab[r] = stab;
       cpar[r] = cs.parent;
     }
