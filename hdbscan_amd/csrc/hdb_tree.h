@@ -449,13 +449,17 @@ __device__ void linkage_gather(const TreeView<Real>& t, int n, int16_t* su, int1
 
 // The linkage run by one wave, 64 edges at a time.  Each lane finds the
 // roots of its edge's endpoints in the union-find state at the start of the
-// batch (with their records); the unions are then applied in edge order,
-// each broadcast from its lane so that later lanes replace a root that just
-// lost by the winner and take the winner's new record.  Every edge therefore
-// sees exactly the roots, sizes and node ids the sequential algorithm sees,
-// and the unions (by size, ties to the first endpoint's root) are the same.
-// Depth stays <= log2(n) without path compression because unions are by
-// size.  Per edge: three lane reads, two compares and two selects per root.
+// batch (with their records); the unions are then applied in edge order.
+// For edge k every lane forms the union of its own edge from its current
+// roots (winner, loser, merged record), lane k stores its union and the
+// three values are broadcast from lane k, so that later lanes replace a root
+// that just lost by the winner and take the winner's new record.  Every edge
+// therefore sees exactly the roots, sizes and node ids the sequential
+// algorithm sees, and the unions (by size, ties to the first endpoint's
+// root) are the same.  Depth stays <= log2(n) without path compression
+// because unions are by size.  Per edge: five vector operations for the
+// union (no scalar unpacking of lane reads), one lane's two stores, three
+// lane reads, two compares and two selects per root.
 template <typename Real>
 __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, const int16_t* sv,
                              uint32_t* rec) {
@@ -476,32 +480,34 @@ __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, 
       kc = rec[c];
     }
     uint32_t oa = 0, oc = 0;  // this lane's records at its own edge
-    const int cnt = min(64, n - 1 - i0);
+    // Wave-uniform edge count: the loop test is a scalar compare.
+    const int cnt = __builtin_amdgcn_readfirstlane(min(64, n - 1 - i0));
     // Wait for the root and record loads here: a wait inside the loop below
     // would also wait, every edge, for the previous edge's LDS stores.
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
     for (int k = 0; k < cnt; ++k) {
-      const uint32_t roots = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)ra | ((uint32_t)rc << 16)), k);
-      const uint32_t ea = (uint32_t)__builtin_amdgcn_readlane((int)ka, k);
-      const uint32_t ec = (uint32_t)__builtin_amdgcn_readlane((int)kc, k);
-      const int kra = (int)(roots & 0xFFFFu), krc = (int)(roots >> 16);
-      const uint32_t sa = ea >> 16, sc = ec >> 16;
-      const int win = sa < sc ? krc : kra, los = sa < sc ? kra : krc;
-      const uint32_t merged = (uint32_t)(n + i0 + k) | ((sa + sc) << 16);
+      // Every lane forms the union of its own edge from its current roots;
+      // lane k's is edge k's (the earlier edges of the batch are applied).
+      const bool cw = (ka >> 16) < (kc >> 16);
+      const int win = cw ? rc : ra, los = cw ? ra : rc;
+      const uint32_t merged = (uint32_t)(n + i0 + lane) | (((ka >> 16) + (kc >> 16)) << 16);
+      if (lane == k) {
+        uf[los] = (int16_t)win;
+        rec[win] = merged;
+      }
+      const int swin = __builtin_amdgcn_readlane(win, k);
+      const int slos = __builtin_amdgcn_readlane(los, k);
+      const uint32_t smerged = (uint32_t)__builtin_amdgcn_readlane((int)merged, k);
       // Lane k's own records hold the edge's children and sizes.
       const bool mine = lane == k;
       oa = mine ? ka : oa;
       oc = mine ? kc : oc;
-      // Every lane stores the same values to the same addresses: no branch
-      // around a one-lane store.
-      uf[los] = (int16_t)win;
-      rec[win] = merged;
-      const bool ma = ra == los || ra == win;
-      const bool mc = rc == los || rc == win;
-      ra = ma ? win : ra;
-      ka = ma ? merged : ka;
-      rc = mc ? win : rc;
-      kc = mc ? merged : kc;
+      const bool ma = ra == slos || ra == swin;
+      const bool mc = rc == slos || rc == swin;
+      ra = ma ? swin : ra;
+      ka = ma ? smerged : ka;
+      rc = mc ? swin : rc;
+      kc = mc ? smerged : kc;
     }
     if (valid) {
       t.left[i] = (int16_t)(oa & 0xFFFFu);
