@@ -125,6 +125,9 @@ template <int DK, int RSET> struct GramShape {
 
 // Candidate slots per row (columns beyond it send the row to the exact path).
 constexpr int kGramCap = 32;
+// +inf norm terms after each window's columns: a 16-column step starting
+// inside the window reads at most 15 past it.
+constexpr int kGramPad = 16;
 
 // Bound constant c and half the absolute term for padded dimension dk.
 __host__ __device__ constexpr float gram_c(int dk) {
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(256) void gram_prep_kernel(const float* x, int64_t 
 template <int DK, int RSET>
 HDB_HD uint32_t gram_lds_bytes(int n) {
   using G = GramShape<DK, RSET>;
-  return (uint32_t)G::TJ * G::KS * 2u + 2u * (uint32_t)n * 4u;
+  return (uint32_t)G::TJ * G::KS * 2u + 2u * (uint32_t)(n + kGramPad) * 4u;
 }
 
 template <int DK> constexpr int gram_max_threads() { return GramShape<DK, 1>::MAXT; }
@@ -212,7 +215,9 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
   int n;
   if (!win.get(blockIdx.y, wbase, n) || row0 >= n) return;
   uint16_t* bcol = reinterpret_cast<uint16_t*>(lds);          // [TJ][KS] bf16
-  float* nbound = reinterpret_cast<float*>(bcol + TJ * KS);   // [2][n]
+  // [2][n + kGramPad]: upper then lower norm terms, +inf past the window.
+  float* nbound = reinterpret_cast<float*>(bcol + TJ * KS);
+  const int np = n + kGramPad;
   const size_t rbase = (size_t)wbase;
 
   if (tid == 0) sh_nmax = 0;
@@ -225,11 +230,12 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
   uint32_t nmax = 0;
   for (int j = tid; j < n; j += T) {
     nbound[j] = nb[rbase + j];
-    nbound[n + j] = nb[points + rbase + j];
+    nbound[np + j] = nb[points + rbase + j];
     const float sq = nb[2 * points + rbase + j];
     // NaN compares false below, so it disables the filter like a huge norm.
     nmax = max(nmax, sq <= 1.2676506002282294e30f ? __float_as_uint(sq) : 0xFFFFFFFFu);
   }
+  if (tid < 2 * kGramPad) nbound[(tid / kGramPad) * np + n + tid % kGramPad] = INFINITY;
   atomicMax(&sh_nmax, nmax);
   __syncthreads();
   if (sh_nmax > __float_as_uint(1.2676506002282294e30f)) {
@@ -272,7 +278,7 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = min(row0 + wave * WR + g * 16 + 4 * grp + r, n - 1);
-        nrow[g][r] = nbound[pass * n + i];
+        nrow[g][r] = nbound[pass * np + i];
 #pragma unroll
         for (int q = 0; q < M; ++q) mn[g][r][q] = INFINITY;
       }
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
       for (int s0 = 0; s0 < cn; s0 += 16) {
         const int cl = s0 + (lane & 15);
         const int j = c0 + cl;
-        const float ncj = j < n ? nbound[pass * n + j] : INFINITY;
+        const float ncj = nbound[pass * np + j];  // j < n + 15: +inf past the window
         gram_f32x4 acc[RS];
 #pragma unroll
         for (int g = 0; g < RS; ++g)
@@ -337,12 +343,15 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               // With mn[0] <= mn[1], the second smallest of {mn[0], mn[1], v}
-              // is their median (one instruction); the smallest is a compare
-              // and select.  No NaN quieting is needed: the filter runs only
-              // on finite bounds (columns past the window add +inf).
+              // is their median (one instruction); the smallest is an unsigned
+              // minimum of the bit patterns (one instruction, where fminf adds
+              // NaN canonicalisation): the upper bounds are positive (>= the
+              // exact sum plus a positive absolute term) or +inf (columns
+              // past the window), and positive floats order like their bits.
+              // No NaN can occur: the filter runs only on finite norms.
               const float v = acc[g][r];
               if constexpr (M == 2) mn[g][r][1] = __builtin_amdgcn_fmed3f(mn[g][r][0], mn[g][r][1], v);
-              mn[g][r][0] = v < mn[g][r][0] ? v : mn[g][r][0];
+              mn[g][r][0] = __uint_as_float(min(__float_as_uint(v), __float_as_uint(mn[g][r][0])));
             }
         } else {
           // Candidates of row (set g, 4 grp + r): the 16 lanes of a group
