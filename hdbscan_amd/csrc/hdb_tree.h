@@ -143,8 +143,10 @@ __device__ void introsort(EdgeKey<Real>* keys, int m, int depth_override, int* s
 // both position lists with ballots and prefix counts, performs the K swaps in
 // parallel and parks the pivot: the same permutation as the serial scan.
 // Spans are independent, so waves process different spans concurrently in
-// rounds; spans of <= 16 keys are insertion-sorted one lane each at the end,
-// spans whose depth ran out are heap-sorted one lane each.
+// rounds, one partition per span and round (breadth first: the longest
+// chain of dependent scans is about 2m keys instead of 4m when a wave keeps
+// partitioning one side); spans of <= 16 keys are insertion-sorted one lane
+// each at the end, spans whose depth ran out are heap-sorted one lane each.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -292,13 +294,14 @@ __device__ void small_sort(EdgeKey<Real>* keys, int lo, int hi) {
   }
 }
 
-// Task lists of the parallel introsort (int16 entries).  Spans pushed in one
+// Task lists of the parallel introsort (int16 entries).  Spans queued for a
 // round are disjoint with >= 8 keys each (the deferred side of a partition of
-// >= 17 keys), so a round holds at most m/8 + 1 tasks; spans left for
-// insertion sort are disjoint with >= 2 keys (at most m/2); spans handed to
-// heapsort are popped tasks (at most m/8 + 1 in total).
+// >= 17 keys, or a kept side of >= 18), so a round holds at most m/8 + 1
+// tasks; spans left for insertion sort are disjoint with >= 2 keys (at most
+// m/2); spans handed to heapsort are popped tasks (at most m/8 + 1 in
+// total).
 struct SortLists {
-  int16_t* cur;    // [3][cap]: lo, hi, depth
+  int16_t* cur;    // [3][cap]: lo, hi, 2 depth + (1 for a kept side)
   int16_t* next;   // [3][cap]
   int16_t* heap;   // [2][cap]: lo, hi
   int16_t* small;  // [2][cap_small]: lo, hi
@@ -322,7 +325,7 @@ __device__ void parallel_introsort(EdgeKey<Real>* keys, int m, int depth_overrid
     if (depth_override > -1000) depth = depth_override;
     L.cur[0] = 0;
     L.cur[L.cap] = (int16_t)(m - 1);
-    L.cur[2 * L.cap] = (int16_t)depth;
+    L.cur[2 * L.cap] = (int16_t)(depth * 2);
     counters[0] = m >= 2 ? 1 : 0;
     counters[1] = 0;
     counters[2] = 0;
@@ -335,8 +338,11 @@ __device__ void parallel_introsort(EdgeKey<Real>* keys, int m, int depth_overrid
     const int ncur = counters[0];
     if (ncur == 0) break;
     for (int t = wave; t < ncur; t += waves) {
-      int lo = cur[t], hi = cur[L.cap + t], depth = cur[2 * L.cap + t];
-      if (depth < 0) {
+      const int lo = cur[t], hi = cur[L.cap + t], field = cur[2 * L.cap + t];
+      int depth = field >> 1;
+      // NumPy tests the depth budget only on spans popped from its stack,
+      // never on the side it keeps partitioning (bit 0: a kept side).
+      if (!(field & 1) && depth < 0) {
         if (lane == 0) {
           const int h = atomicAdd(&counters[3], 1);
           L.heap[h] = (int16_t)lo;
@@ -344,23 +350,34 @@ __device__ void parallel_introsort(EdgeKey<Real>* keys, int m, int depth_overrid
         }
         continue;
       }
-      while (hi - lo > kSmallSort) {
+      if (hi - lo > kSmallSort) {
+        // One partition per span and round: both sides go to the next round
+        // (breadth first), NumPy's pushed side and its kept side alike.
         const int i = wave_partition(keys, lo, hi, lpos, rpos, lane);
         --depth;
-        int plo, phi;
+        int plo, phi, klo, khi;
         if (i - lo < hi - i) {
-          plo = i + 1; phi = hi; hi = i - 1;
+          plo = i + 1; phi = hi; klo = lo; khi = i - 1;
         } else {
-          plo = lo; phi = i - 1; lo = i + 1;
+          plo = lo; phi = i - 1; klo = i + 1; khi = hi;
         }
         if (lane == 0) {
           const int q = atomicAdd(&counters[1], 1);
           next[q] = (int16_t)plo;
           next[L.cap + q] = (int16_t)phi;
-          next[2 * L.cap + q] = (int16_t)depth;
+          next[2 * L.cap + q] = (int16_t)(depth * 2);
+          if (khi - klo > kSmallSort) {
+            const int r = atomicAdd(&counters[1], 1);
+            next[r] = (int16_t)klo;
+            next[L.cap + r] = (int16_t)khi;
+            next[2 * L.cap + r] = (int16_t)(depth * 2 + 1);
+          } else if (khi > klo) {
+            const int r = atomicAdd(&counters[2], 1);
+            L.small[r] = (int16_t)klo;
+            L.small[L.cap_small + r] = (int16_t)khi;
+          }
         }
-      }
-      if (hi > lo && lane == 0) {
+      } else if (hi > lo && lane == 0) {
         const int q = atomicAdd(&counters[2], 1);
         L.small[q] = (int16_t)lo;
         L.small[L.cap_small + q] = (int16_t)hi;
