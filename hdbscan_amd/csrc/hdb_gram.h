@@ -434,11 +434,68 @@ __device__ __forceinline__ float exact_sq(const float* xi, const float* xj, int 
 // matrix when one is materialised) and keeps the KMAX smallest; a row
 // marked for the exact path streams every column instead.
 // Grid: (ceil(n / 256), windows).
+// Exact selection for one row: the ms-th smallest coordinate-order squared
+// distance over its candidates (at most kGramCap), or over the whole window
+// when the row has more (ties, or the filter off for the window).  xb: the
+// window's points (device memory or LDS), di: the row of a materialised
+// matrix (sqrt values) or null.  Sets bad for a non-finite full-row value.
+template <int KMAX>
+__device__ __forceinline__ float exact_row(const float* xb, const float* di, int n, int d, int ms, int i,
+                                           const uint16_t* cr, int m, int& bad) {
+  constexpr int CAP = kGramCap;
+  const float* xi = xb + (size_t)i * d;
+  float lst[KMAX];
+#pragma unroll
+  for (int q = 0; q < KMAX; ++q) lst[q] = INFINITY;
+  if (m <= CAP) {
+    // Four candidates at a time, their reads in flight together (the K
+    // smallest values do not depend on the order they are offered in).  The
+    // list's slots are 64-byte aligned, so four entries are one 8-byte load.
+    int p = 0;
+    for (; p + 4 <= m; p += 4) {
+      const uint2 jj = *reinterpret_cast<const uint2*>(cr + p);
+      const int j[4] = {(int)(jj.x & 0xFFFFu), (int)(jj.x >> 16), (int)(jj.y & 0xFFFFu), (int)(jj.y >> 16)};
+      float t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = di ? di[j[u]] : exact_sq(xi, xb + (size_t)j[u] * d, d);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) list_offer<KMAX>(lst, t[u]);
+    }
+    for (; p < m; ++p) {
+      const int j = cr[p];
+      // The matrix holds sqrt(s); squares of its values order like s.
+      const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
+      list_offer<KMAX>(lst, t);
+    }
+  } else {
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+      float t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = di ? di[j + u] : exact_sq(xi, xb + (size_t)(j + u) * d, d);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        bad |= !isfinite(t[u]);
+        list_offer<KMAX>(lst, t[u]);
+      }
+    }
+    for (; j < n; ++j) {
+      const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
+      bad |= !isfinite(t);
+      list_offer<KMAX>(lst, t);
+    }
+  }
+  float t = lst[0];
+#pragma unroll
+  for (int q = 1; q < KMAX; ++q)
+    if (q == ms - 1) t = lst[q];
+  return di ? t : sqrt(t);
+}
+
 template <int KMAX>
 __global__ __launch_bounds__(256) void core_exact_kernel(const float* x, const float* dist, Windows win, int d,
                                                          int ms, const uint16_t* cand, const int32_t* ncand,
                                                          float* core, int32_t* status) {
-  constexpr int CAP = kGramCap;
   int64_t base;
   int n;
   if (!win.get(blockIdx.y, base, n) || (int)(blockIdx.x * 256) >= n) return;
@@ -446,58 +503,40 @@ __global__ __launch_bounds__(256) void core_exact_kernel(const float* x, const f
   int bad = 0;
   if (i < n) {
     const int64_t row = base + i;
-    const float* xb = x + (size_t)base * d;
-    const float* xi = xb + (size_t)i * d;
     const float* di = dist ? dist + (size_t)row * n : nullptr;
-    const int m = ncand[row];
-    float lst[KMAX];
-#pragma unroll
-    for (int q = 0; q < KMAX; ++q) lst[q] = INFINITY;
-    if (m <= CAP) {
-      // Four candidates at a time, their gathers in flight together (the
-      // K smallest values do not depend on the order they are offered in).
-      // The list's slots are 64-byte aligned, so four entries are one
-      // 8-byte load.
-      const uint16_t* cr = cand + row * CAP;
-      int p = 0;
-      for (; p + 4 <= m; p += 4) {
-        const uint2 jj = *reinterpret_cast<const uint2*>(cr + p);
-        const int j[4] = {(int)(jj.x & 0xFFFFu), (int)(jj.x >> 16), (int)(jj.y & 0xFFFFu), (int)(jj.y >> 16)};
-        float t[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) t[u] = di ? di[j[u]] : exact_sq(xi, xb + (size_t)j[u] * d, d);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) list_offer<KMAX>(lst, t[u]);
-      }
-      for (; p < m; ++p) {
-        const int j = cr[p];
-        // The matrix holds sqrt(s); squares of its values order like s.
-        const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
-        list_offer<KMAX>(lst, t);
-      }
-    } else {
-      int j = 0;
-      for (; j + 4 <= n; j += 4) {
-        float t[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) t[u] = di ? di[j + u] : exact_sq(xi, xb + (size_t)(j + u) * d, d);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          bad |= !isfinite(t[u]);
-          list_offer<KMAX>(lst, t[u]);
-        }
-      }
-      for (; j < n; ++j) {
-        const float t = di ? di[j] : exact_sq(xi, xb + (size_t)j * d, d);
-        bad |= !isfinite(t);
-        list_offer<KMAX>(lst, t);
-      }
-    }
-    float t = lst[0];
-#pragma unroll
-    for (int q = 1; q < KMAX; ++q)
-      if (q == ms - 1) t = lst[q];
-    core[row] = di ? t : sqrt(t);
+    core[row] = exact_row<KMAX>(x + (size_t)base * d, di, n, d, ms, i, cand + row * kGramCap, ncand[row], bad);
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+}
+
+// LDS bytes above which core_exact_lds_kernel is not used.
+constexpr uint32_t kExactLdsBytes = 65536;
+
+// The same selection with the window's points staged in LDS first (one
+// workgroup per window, n * d floats <= kExactLdsBytes): every candidate is
+// read from LDS instead of gathered from device memory.
+template <int KMAX>
+__global__ __launch_bounds__(1024) void core_exact_lds_kernel(const float* x, Windows win, int d, int ms,
+                                                             const uint16_t* cand, const int32_t* ncand,
+                                                             float* core, int32_t* status) {
+  extern __shared__ __align__(16) float xs[];
+  int64_t base;
+  int n;
+  if (!win.get(blockIdx.x, base, n)) return;
+  const float* xb = x + (size_t)base * d;
+  const int total = n * d;
+  if (((((uintptr_t)xb) & 15) == 0) && (total & 3) == 0) {
+    const float4* src = reinterpret_cast<const float4*>(xb);
+    float4* dst = reinterpret_cast<float4*>(xs);
+    for (int k = threadIdx.x; k < total / 4; k += blockDim.x) dst[k] = src[k];
+  } else {
+    for (int k = threadIdx.x; k < total; k += blockDim.x) xs[k] = xb[k];
+  }
+  __syncthreads();
+  int bad = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int64_t row = base + i;
+    core[row] = exact_row<KMAX>(xs, nullptr, n, d, ms, i, cand + row * kGramCap, ncand[row], bad);
   }
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
 }

@@ -400,6 +400,16 @@ int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist,
   const int r = ms <= 3 ? launch_gram_d<1>(s, P, L, ms, g) : launch_gram_d<2>(s, P, L, ms, g);
   if (r) return r;
   stage_mark(s, "core_gram_kernel");
+  const uint64_t xbytes = (uint64_t)L.nmax * d * sizeof(float);
+  if (!dist && xbytes <= kExactLdsBytes) {
+    // The window's points fit in LDS: one workgroup per window reads them
+    // once and every candidate from LDS.
+    const int threads = (int)std::min<int64_t>(1024, (L.nmax + 63) / 64 * 64);
+    auto fn = ms <= 8 ? core_exact_lds_kernel<8> : core_exact_lds_kernel<16>;
+    if (int r = allow_lds((const void*)fn, (uint32_t)xbytes)) return r;
+    fn<<<(unsigned)L.count, threads, (uint32_t)xbytes, s>>>(x, L.win, (int)d, ms, g.cand, g.ncand, core, status);
+    return check_launch("core_exact_kernel");
+  }
   const dim3 grid((unsigned)((L.nmax + 255) / 256), (unsigned)L.count);
   if (ms <= 8)
     core_exact_kernel<8><<<grid, 256, 0, s>>>(x, dist, L.win, (int)d, ms, g.cand, g.ncand, core, status);
