@@ -397,6 +397,72 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
   }
 }
 
+// The same outputs for DK <= 16 with a block's 256 points staged through
+// LDS: the rows are read and the bf16 pieces written as contiguous 16-byte
+// vectors across lanes (one point per lane would touch 64 cache lines per
+// instruction); the per-point arithmetic is that of gram_prep_kernel.
+template <int DK>
+__global__ __launch_bounds__(256) void gram_prep_lds_kernel(const float* x, int64_t points, int d, uint16_t* pa,
+                                                            uint16_t* pb, float* nb) {
+  static_assert(DK <= 16, "LDS staging sized for DK <= 16");
+  __shared__ __align__(16) float xin[256 * DK];
+  __shared__ __align__(16) uint16_t oa[256 * 2 * DK];
+  __shared__ __align__(16) uint16_t ob[256 * 2 * DK];
+  const int t = threadIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.x * 256;
+  const int cnt = (int)min((int64_t)256, points - p0);
+  const float* xb = x + (size_t)p0 * d;
+  const int total = cnt * d;  // d <= DK
+  if (((((uintptr_t)xb) & 15) == 0) && (total & 3) == 0) {
+    for (int k = t; k < total / 4; k += 256) reinterpret_cast<float4*>(xin)[k] = reinterpret_cast<const float4*>(xb)[k];
+  } else {
+    for (int k = t; k < total; k += 256) xin[k] = xb[k];
+  }
+  __syncthreads();
+  if (t < cnt) {
+    const float c = gram_c(DK);
+    float sq = 0.f;
+    uint16_t* ap = oa + t * 2 * DK;
+    uint16_t* bp = ob + t * 2 * DK;
+#pragma unroll
+    for (int k0 = 0; k0 < DK; k0 += 8) {
+      float v[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = k0 + e < d ? xin[t * d + k0 + e] : 0.f;
+      gram_s16x8 ah, al, bh, bl;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        sq = fmaf(v[e], v[e], sq);
+        const uint32_t h = bf16_bits(v[e]);
+        ah[e] = (short)h;
+        al[e] = (short)bf16_bits(v[e] - bf16_value(h));
+        const float w = -2.f * v[e];
+        const uint32_t g = bf16_bits(w);
+        bh[e] = (short)g;
+        bl[e] = (short)bf16_bits(w - bf16_value(g));
+      }
+      *reinterpret_cast<gram_s16x8*>(ap + k0) = ah;
+      *reinterpret_cast<gram_s16x8*>(ap + DK + k0) = al;
+      *reinterpret_cast<gram_s16x8*>(bp + k0) = bh;
+      *reinterpret_cast<gram_s16x8*>(bp + DK + k0) = bl;
+    }
+    const int64_t p = p0 + t;
+    nb[p] = fmaf(sq, c, sq) + kGramEh;
+    nb[points + p] = fmaf(sq, -c, sq) - kGramEh;
+    nb[2 * points + p] = sq;
+  }
+  __syncthreads();
+  // cnt points of 2 DK bf16 values per array: cnt * DK / 4 vectors of 16 B
+  // (the block's first point is 256 * 4 DK bytes into each array).
+  const int vecs = cnt * DK / 4;
+  uint4* ga = reinterpret_cast<uint4*>(pa + (size_t)p0 * 2 * DK);
+  uint4* gb = reinterpret_cast<uint4*>(pb + (size_t)p0 * 2 * DK);
+  for (int k = t; k < vecs; k += 256) {
+    ga[k] = reinterpret_cast<const uint4*>(oa)[k];
+    gb[k] = reinterpret_cast<const uint4*>(ob)[k];
+  }
+}
+
 // Exact coordinate-order squared distance of rows xi and xj (d floats), 16
 // loads in flight per operand; 16-byte loads when both rows are aligned.
 __device__ __forceinline__ float exact_sq(const float* xi, const float* xj, int d) {

@@ -376,8 +376,8 @@ int launch_gram_prep(hipStream_t s, const Plan& P, const float* x, int64_t point
   const GramBuffers g = gram_buffers(P, ws);
   const unsigned blocks = (unsigned)((points + 255) / 256);
   switch (P.gram_dk) {
-    case 8: gram_prep_kernel<8><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
-    case 16: gram_prep_kernel<16><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    case 8: gram_prep_lds_kernel<8><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    case 16: gram_prep_lds_kernel<16><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
     case 32: gram_prep_kernel<32><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
     case 64: gram_prep_kernel<64><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
     case 128: gram_prep_kernel<128><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
