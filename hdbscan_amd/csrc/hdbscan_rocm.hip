@@ -237,7 +237,10 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   int vpt = want_vpt == 8 ? 0 : want_vpt;
   if (!vpt) {
     vpt = n <= 512 ? 1 : n <= 1024 ? 2 : 4;
-    if (vpt == 2 && P.core && dk > 0 && (int64_t)dk * rs <= 64) vpt = 1;
+    // Points path up to 1024 with at most 64 bytes of coordinates: one slot
+    // per thread and 1024 threads beat two slots and 512 threads (1.5 % at
+    // N=1024, D=16 with core distances; 2.4 % at D=8 without).
+    if (vpt == 2 && dk > 0 && (int64_t)dk * rs <= 64) vpt = 1;
     // Points path up to 2048: two slots per thread and 1024 threads beat
     // four slots and 512 threads (20 % at N=2048, D=16).
     if (vpt == 4 && n <= 2048 && dk > 0 && (int64_t)2 * dk * rs <= 64 * 4) vpt = 2;
