@@ -56,8 +56,8 @@ extern "C" {
  * matrix path only); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
  * 3 -> 4; 0 = 2; clamped to what the dimension allows).
- * Bits 14..15: Prim repacking interval of the points path (0 -> 64 steps,
- * 1 -> never, 2 -> 32, 3 -> 128). */
+ * Bits 14..15: Prim repacking interval of the points path (0 -> 128 steps,
+ * 1 -> never, 2 -> 32, 3 -> 64). */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int hdb_version(void);
