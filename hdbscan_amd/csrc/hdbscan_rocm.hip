@@ -158,10 +158,14 @@ int pad_dims(int64_t d) {
 
 // Row sets per wave of the Gram core kernel: 2 or 4 up to 32 dimensions,
 // 1 or 2 up to 128, 1 beyond (register budget).  Flag bits 12..13 pick one
-// (1 -> 1, 2 -> 2, 3 -> 4); 0 chooses 2.
-int gram_rsets(int dk, int flags) {
+// (1 -> 1, 2 -> 2, 3 -> 4); 0 chooses by window size (below).
+int gram_rsets(int dk, int64_t n, int flags) {
   const int want = (flags >> 12) & 3;
-  const int rs = want == 1 ? 1 : want == 3 ? 4 : 2;
+  // Default: two 16-row sets per wave; four for windows of 1025..2048 points
+  // (each column chunk is read by half as many row blocks; measured on
+  // ragged batches: 12.66 -> 11.89 ms per call, while four sets for the
+  // windows above 2048 points, the batch's critical path, cost 5 %).
+  const int rs = want == 1 ? 1 : want == 3 ? 4 : want == 2 ? 2 : (n > 1024 && n <= 2048) ? 4 : 2;
   if (dk <= 32) return rs == 1 ? 2 : rs;
   if (dk <= 128) return rs == 4 ? 2 : rs;
   return 1;
@@ -195,7 +199,7 @@ void set_gram(Plan& P, int64_t n, int64_t d, int32_t ms, uint32_t lds_limit) {
   // away; the filter then only pays when asked for.
   if (P.matrix && !(P.flags & HDB_FLAG_FILTER)) return;
   const int dk = d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
-  const int rs = gram_rsets(dk, P.flags);
+  const int rs = gram_rsets(dk, n, P.flags);
   int waves = (dk >= 64 ? 256 : 512) / 64;
   const int wr = 16 * rs;  // rows per wave
   while (waves > 1 && (int64_t)(waves / 2) * wr >= n) waves /= 2;

@@ -55,7 +55,8 @@ extern "C" {
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4; 8 on the speculative
  * matrix path only); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
- * 3 -> 4; 0 = 2; clamped to what the dimension allows).
+ * 3 -> 4; 0 = 4 for windows of 1025..2048 points, else 2; clamped to what
+ * the dimension allows).
  * Bits 14..15: Prim repacking interval of the points path (0 -> 128 steps,
  * 1 -> never, 2 -> 32, 3 -> 64). */
 
