@@ -37,16 +37,22 @@ def hdbscan(
     backend="rocm",
     precision="float32",
 ):
-    """Return device labels for Euclidean EOM HDBSCAN, excluding the root.
+    """Cluster each window of ``X`` on the GPU and return its labels.
 
-    Accepts [N, D] or [B, N, D]; min_samples includes the point itself.
-    Host arrays are explicitly transferred.  Native ROCm tensors are never
-    copied to host.  One status word is read back after the kernels to report
-    non-finite inputs or overflowing distances; clustering intermediates stay
-    on the GPU.  float32 and float64 are supported.  Windows are limited to
-    4096 points and 2*B*N^2*itemsize to 512 MiB.  Float32 rounding can change
-    near-tied hierarchies relative to scikit-learn's float64 calculation.
-    Labels are int32 torch tensors on the GPU, with -1 for noise.
+    ``X`` is one window ``[N, D]`` or a batch ``[B, N, D]`` of independent
+    windows, either a ROCm ``torch.Tensor`` (used in place, never copied to
+    the host) or host data (NumPy arrays, nested lists, CPU tensors), which is
+    copied to the current GPU once.  The clustering is Euclidean HDBSCAN with
+    excess-of-mass selection; the root cluster is never selected, and a
+    point's own position counts towards ``min_samples``.  The result is a new
+    ``int32`` tensor ``[B, N]`` (``[N]`` for one window) on the same GPU,
+    ``-1`` marking noise, enqueued on the current stream.  The only
+    host synchronisation is one status word, which turns a non-finite input
+    or an overflowing distance into a ``ValueError``.  Work happens in
+    ``precision`` (float32 or float64); in float32, pairs whose distances tie
+    or nearly tie can order differently than scikit-learn's float64 run.
+    Limits: 4096 points per window, and ``2 * B * N**2 * itemsize`` at most
+    512 MiB per call (the admission rule of the reference API).
     """
     if backend not in BACKENDS:
         raise ValueError("backend must be 'rocm'")
@@ -80,8 +86,14 @@ def hdbscan(
             raise ValueError("X must contain real numeric values")
         shape = tuple(X.shape)
     else:
-        if type(X).__module__.split(".")[0] == "torch":
+        module = type(X).__module__.split(".")[0]
+        if module == "torch":
             X = X.detach().numpy()
+        elif hasattr(X, "__cuda_array_interface__") or (
+                hasattr(X, "__dlpack_device__") and not isinstance(X, np.ndarray)):
+            # Device arrays of other frameworks are not read through the host
+            # (reference hdbscan_gpu/_api.py:62-64).
+            raise TypeError("use a native ROCm torch tensor or a NumPy host array")
         X = np.asarray(X)
         if X.dtype.kind not in "fiu":
             raise ValueError("X must contain real numeric values")

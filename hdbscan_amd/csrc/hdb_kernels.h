@@ -24,6 +24,10 @@ __global__ __launch_bounds__(256) void check_finite_kernel(const Real* x, size_t
 // reference's triangle-and-mirror kernel writes (reference
 // hdbscan_gpu/_kernels.py:45-73).  Grid: (nt (nt + 1) / 2, batch) for nt
 // tiles per side; block t is the t-th tile of the upper triangle in row order.
+// Input check: the diagonal tiles stage every coordinate of the window
+// exactly once, so they OR HDB_STATUS_NONFINITE (2) into *status for a
+// non-finite coordinate (the reference's input check, hdbscan_gpu/_cuda.py:
+// 96-99, which outranks the overflow bit 1 set for a non-finite distance).
 template <typename Real>
 __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int d, Real* out, int32_t* status) {
   constexpr int TILE = 64, KC = 16;
@@ -40,6 +44,8 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
   const int ti = rt * TILE, tj = (rt + t) * TILE;
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const Real* xb = x + (size_t)b * n * d;
+  const bool diag = ti == tj;
+  int nonfinite = 0;
   Real tot[4][4];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -50,7 +56,9 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
     for (int idx = tid; idx < TILE * KC; idx += 256) {
       const int row = idx / KC, kk = idx - row * KC, k = k0 + kk;
       const int gi = ti + row, gj = tj + row;
-      si[kk][row] = (gi < n && k < d) ? xb[(size_t)gi * d + k] : Real(0);
+      const Real vi = (gi < n && k < d) ? xb[(size_t)gi * d + k] : Real(0);
+      nonfinite |= diag && !isfinite(vi);
+      si[kk][row] = vi;
       sj[kk][row] = (gj < n && k < d) ? xb[(size_t)gj * d + k] : Real(0);
     }
     __syncthreads();
@@ -97,6 +105,7 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
     }
   }
   if (__any(bad) && (tid & 63) == 0) atomicOr(status, 1);
+  if (__any(nonfinite) && (tid & 63) == 0) atomicOr(status, 2);
 }
 
 // Upper bound of b*b (see square_bound in hdb_prim.h).

@@ -15,7 +15,9 @@
 // Reference semantics: hdbscan_gpu/_kernels.py:164-320 (and sklearn
 // _linkage.pyx:111-224).  Output edges are (source, next, weight) in step
 // order, with source 0 when the weight is not below REAL_MAX
-// (reference hdbscan_gpu/_kernels.py:307-317).
+// (reference hdbscan_gpu/_kernels.py:307-317).  The update and tie rules are
+// scikit-learn's (BSD-3; notices in THIRD_PARTY_NOTICES.md at the repository
+// root).
 #pragma once
 
 #include "hdb_common.h"

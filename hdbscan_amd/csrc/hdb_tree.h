@@ -13,6 +13,9 @@
 //   condense  reference hdbscan_gpu/_kernels.py:558-643
 //   eom       reference hdbscan_gpu/_kernels.py:671-709
 //   labels    reference hdbscan_gpu/_kernels.py:712-760
+// heap_sort / introsort follow NumPy's scalar aheapsort / aquicksort step for
+// step, and the tree semantics are scikit-learn's _tree.pyx (BSD-3; notices
+// in THIRD_PARTY_NOTICES.md at the repository root).
 #pragma once
 
 #include "hdb_common.h"

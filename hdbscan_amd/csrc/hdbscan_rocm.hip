@@ -17,6 +17,7 @@
 //   norm terms (filter) | Prim repacking records (points path)
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_set>
@@ -702,39 +703,44 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
   return 0;
 }
 
-// Internal streams (default priority) and fork/join events, per device.
+// Internal streams (default priority) and fork/join events, one set per
+// (device, caller stream): ragged calls on different caller streams get
+// different internal streams and may overlap; calls on one caller stream
+// serialise on its set, as they would on the stream itself.
 struct SideStreams {
-  bool ready = false;
+  std::mutex mu;  // held while a call enqueues on this set
   hipStream_t s[kSideStreams];
   hipEvent_t fork, head, join[kSideStreams];
 };
-std::mutex g_side_mutex;
-SideStreams g_side[64];
 
-int side_streams(SideStreams** out) {
+int side_streams(hipStream_t caller, SideStreams** out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, SideStreams*> sets;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail("no current HIP device");
-  SideStreams& S = g_side[dev];
-  if (!S.ready) {
+  if (hipGetDevice(&dev) != hipSuccess) return fail("no current HIP device");
+  std::lock_guard<std::mutex> lock(mu);
+  SideStreams*& S = sets[{dev, caller}];
+  if (!S) {
+    SideStreams* fresh = new SideStreams;
     for (int i = 0; i < kSideStreams; ++i)
-      if (hipStreamCreateWithFlags(&S.s[i], hipStreamNonBlocking) != hipSuccess ||
-          hipEventCreateWithFlags(&S.join[i], hipEventDisableTiming) != hipSuccess)
+      if (hipStreamCreateWithFlags(&fresh->s[i], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&fresh->join[i], hipEventDisableTiming) != hipSuccess)
         return fail("cannot create internal streams");
-    if (hipEventCreateWithFlags(&S.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&S.head, hipEventDisableTiming) != hipSuccess)
+    if (hipEventCreateWithFlags(&fresh->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&fresh->head, hipEventDisableTiming) != hipSuccess)
       return fail("cannot create events");
-    S.ready = true;
+    S = fresh;
   }
-  *out = &S;
+  *out = S;
   return 0;
 }
 
 template <typename Real>
 int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t* off_dev, int64_t d,
                long long mcs, int ms, double eps, int32_t* labels, unsigned char* ws, int32_t* status) {
-  std::lock_guard<std::mutex> lock(g_side_mutex);
   SideStreams* S = nullptr;
-  if (int r = side_streams(&S)) return r;
+  if (int r = side_streams(s, &S)) return r;
+  std::lock_guard<std::mutex> lock(S->mu);
   const bool timers = g_timers.on;
   g_timers.on = false;  // stage events assume one stream
   int err = 0;
@@ -869,15 +875,9 @@ int hdb_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t
   Plan P;
   if (make_plan(batch, n, d, dtype, 1, 0, &P)) return -1;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const size_t count = (size_t)batch * n * d;
-  const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 4096);
-  if (dtype == HDB_F32) {
-    check_finite_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float*>(x), count, status);
-    if (int r = check_launch("check_finite_kernel")) return r;
+  // distance_kernel checks every input coordinate (bit 2) on its diagonal tiles.
+  if (dtype == HDB_F32)
     return launch_distances<float>(s, static_cast<const float*>(x), batch, n, d, static_cast<float*>(out), status);
-  }
-  check_finite_kernel<double><<<blocks, 256, 0, s>>>(static_cast<const double*>(x), count, status);
-  if (int r = check_launch("check_finite_kernel")) return r;
   return launch_distances<double>(s, static_cast<const double*>(x), batch, n, d, static_cast<double*>(out), status);
 }
 
@@ -900,6 +900,15 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
   hipStream_t s = static_cast<hipStream_t>(stream);
   unsigned char* ws = static_cast<unsigned char*>(workspace);
   const Launch L = dense_launch(batch, n);
+  // The core kernels check distances (bit 1); the input check (bit 2) is a
+  // separate pass here, as no Prim kernel follows.
+  const size_t count = (size_t)batch * n * d;
+  const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 4096);
+  if (dtype == HDB_F32)
+    check_finite_kernel<float><<<blocks, 256, 0, s>>>(static_cast<const float*>(x), count, status);
+  else
+    check_finite_kernel<double><<<blocks, 256, 0, s>>>(static_cast<const double*>(x), count, status);
+  if (int r = check_launch("check_finite_kernel")) return r;
   if (dtype == HDB_F32)
     return launch_core<float>(s, P, static_cast<const float*>(x), L, d, min_samples, static_cast<float*>(core),
                               reinterpret_cast<float*>(ws + P.off_matrix), ws, status, false, true);

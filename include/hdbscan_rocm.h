@@ -127,7 +127,8 @@ int hdb_core_distances(void *stream, const void *x, int64_t batch, int64_t n,
  * device memory (read by the kernels).  Labels are written per point into
  * labels[total], numbered within each window as hdb_cluster numbers them.
  * Windows are grouped by size; the groups run concurrently on internal
- * streams forked from and joined into `stream`.  Windows above 1024 points
+ * streams forked from and joined into `stream` (one set of internal streams
+ * per caller stream, so ragged calls on different streams may overlap).  Windows above 1024 points
  * need d <= 16 (float32) or d <= 8 (float64) (the points path).
  * The reference API is dense only (hdbscan_gpu/_api.py:71-78: one [B, N, D]
  * array); this entry serves uneven windows in one call. */

@@ -15,6 +15,9 @@
  *   condense   reference hdbscan_gpu/_kernels.py:558-643
  *   eom        reference hdbscan_gpu/_kernels.py:671-709
  *   labels     reference hdbscan_gpu/_kernels.py:712-760
+ * The stages follow scikit-learn's _linkage.pyx / _tree.pyx and NumPy's
+ * scalar aquicksort / aheapsort step for step (BSD-3; notices in
+ * THIRD_PARTY_NOTICES.md at the repository root).
  */
 
 #define ORC_CAT2(a, b) a##b

@@ -16,6 +16,11 @@ Fixtures (inputs + expected outputs only):
                    tests use
   workloads.npz    windows of each benchmark workload with scikit-learn labels
   argsort.npz      tie-heavy arrays with np.argsort(kind="quicksort") orders
+  heapsort.npz     the same arrays (sizes above 16) with np.argsort(kind="heapsort")
+                   orders: NumPy's heapsort is the fallback of its introsort
+                   once the depth budget is spent
+
+    python tests/golden/make_golden.py heapsort   writes heapsort.npz only
 """
 
 from __future__ import annotations
@@ -202,6 +207,21 @@ def argsort_cases():
     return records
 
 
+def heapsort_cases():
+    """NumPy's indirect heapsort on the argsort patterns (more than 16 keys:
+    shorter spans never reach the heapsort fallback of the introsort)."""
+    import numpy as np
+
+    records = []
+    for rec in argsort_cases():
+        if len(rec["w64"]) <= 16:
+            continue
+        a = rec["w64"].astype(np.float32 if rec["is32"] else np.float64)
+        records.append({"name": rec["name"], "w64": rec["w64"], "is32": rec["is32"],
+                        "order": np.argsort(a, kind="heapsort").astype(np.int64)})
+    return records
+
+
 def workload_cases():
     import numpy as np
 
@@ -230,6 +250,10 @@ def workload_cases():
 def child():
     import numpy as np
     import sklearn
+
+    np.savez_compressed(os.path.join(HERE, "heapsort.npz"), **pack(heapsort_cases()))
+    if os.environ.get("HDB_GOLDEN_ONLY") == "heapsort":
+        return
 
     info = {"sklearn": sklearn.__version__, "numpy": np.__version__,
             "NPY_DISABLE_CPU_FEATURES": os.environ.get("NPY_DISABLE_CPU_FEATURES")}
@@ -288,4 +312,6 @@ if __name__ == "__main__":
         child()
     else:
         env = dict(os.environ, NPY_DISABLE_CPU_FEATURES=DISABLE, HDB_GOLDEN_CHILD="1")
+        if sys.argv[1:] == ["heapsort"]:
+            env["HDB_GOLDEN_ONLY"] = "heapsort"
         sys.exit(subprocess.run([sys.executable, os.path.abspath(__file__)], env=env).returncode)

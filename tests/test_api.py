@@ -99,3 +99,40 @@ def test_import_does_not_load_torch_or_the_library():
                          cwd=__import__("os").path.dirname(__import__("os").path.dirname(__file__)))
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip() == "ok"
+
+
+class _ForeignDeviceArray:
+    """Stands for another framework's device array (CuPy, JAX, Numba ...)."""
+
+    def __init__(self, protocol):
+        if protocol == "cuda":
+            self.__cuda_array_interface__ = {"shape": (6, 1), "typestr": "<f4", "data": (0, False),
+                                             "version": 3}
+        else:
+            self.__dlpack_device__ = lambda: (2, 0)
+
+    def __array__(self, *args, **kwargs):
+        raise AssertionError("a foreign device array must not be read through the host")
+
+
+@pytest.mark.parametrize("protocol", ["cuda", "dlpack"])
+def test_foreign_device_arrays_rejected_without_host_copy(monkeypatch, protocol):
+    """reference tests: a non-native object exposing a device protocol raises
+    TypeError before any conversion (reference hdbscan_gpu/_api.py:62-64)."""
+    original_import = builtins.__import__
+
+    def no_backend_import(name, *args, **kwargs):
+        if name in ("_rocm", "hdbscan_amd._rocm"):
+            raise AssertionError(f"Foreign input dispatched to {name}")
+        return original_import(name, *args, **kwargs)
+
+    monkeypatch.setattr(builtins, "__import__", no_backend_import)
+    with pytest.raises(TypeError, match="native ROCm torch tensor or a NumPy host array"):
+        hdbscan(_ForeignDeviceArray(protocol), min_cluster_size=2, min_samples=1)
+
+
+def test_numpy_arrays_with_dlpack_are_host_data():
+    """NumPy arrays expose __dlpack_device__ too; they stay host input."""
+    assert hasattr(np.zeros(1), "__dlpack_device__")
+    with pytest.raises(ValueError, match="finite"):
+        hdbscan(np.full((6, 2), np.nan), min_cluster_size=2, min_samples=1)

@@ -275,3 +275,56 @@ def test_custom_op_reports_nonfinite(torch):
     x[0, 3, 1] = float("nan")
     with pytest.raises(ValueError, match="finite"):
         torch.ops.hdbscan_amd.cluster(x, 2, 2, 0.0, "float32")
+
+
+# Shapes that take the materialised-matrix path (float32 beyond 64
+# dimensions; float64 with 32 dimensions above 1024 points): the input check
+# must report a non-finite coordinate as such there too, and it must outrank
+# the overflow that the same coordinate causes in the distances
+# (reference tests/test_api.py:140-174, tests/test_kernels.py:703-715).
+MATRIX_SHAPES = [("float32", 300, 128), ("float64", 2048, 32)]
+
+
+# 1e40 is finite in float64 and +inf once converted to float32.
+NONFINITE_CASES = [(p, n, d, v) for p, n, d in MATRIX_SHAPES
+                   for v in [float("nan"), float("inf"), -float("inf")] + (["1e40"] if p == "float32" else [])]
+
+
+@pytest.mark.parametrize("precision,n,d,value", NONFINITE_CASES)
+@pytest.mark.parametrize("entry", ["hdbscan", "hdb_cluster"])
+def test_matrix_path_nonfinite_reported_as_input(torch, precision, n, d, value, entry):
+    import gpu_helpers as G
+
+    rng = np.random.default_rng(n + d)
+    x = rng.normal(size=(2, n, d))
+    # An overflowing pair in the first window, the non-finite coordinate in
+    # the last row of the second (the last tile and the last chunk of
+    # dimensions).
+    x[0, 0, 0] = -np.finfo(precision).max / 4
+    x[0, 1, 0] = np.finfo(precision).max / 4
+    if value == "1e40":
+        x[1, -1, -1] = 1e40
+    else:
+        x[1, -1, -1] = value
+    if entry == "hdbscan":
+        xd = torch.from_numpy(x).to("cuda")  # float64 on the device, converted by the backend
+        with pytest.raises(ValueError, match="only finite values in the requested precision"):
+            run(xd, min_cluster_size=5, min_samples=5, precision=precision)
+    else:
+        with np.errstate(over="ignore"):
+            xr = x.astype(precision)
+        _, status = G.cluster(xr, 5, 5, 0.0, precision)
+        assert status & 2, status
+        # The first window alone overflows only.
+        _, status = G.cluster(xr[:1], 5, 5, 0.0, precision)
+        assert status == 1
+
+
+@pytest.mark.parametrize("precision,n,d", MATRIX_SHAPES)
+def test_matrix_path_overflow_alone(torch, precision, n, d):
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(n, d)).astype(precision)
+    x[0, 0] = -np.finfo(precision).max / 4
+    x[1, 0] = np.finfo(precision).max / 4
+    with pytest.raises(ValueError, match="overflow"):
+        run(torch.from_numpy(x).to("cuda"), min_cluster_size=5, min_samples=5, precision=precision)

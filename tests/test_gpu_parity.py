@@ -66,7 +66,8 @@ def gram_core_windows():
     stress windows of the distance prefilter."""
     rng = np.random.default_rng(4242)
     out = {f"stress-{k}": v for k, v in prefilter_windows("float32").items()}
-    for n, d in [(3, 2), (5, 4), (17, 8), (77, 16), (1000, 16), (1024, 16), (600, 128), (333, 200), (130, 256)]:
+    for n, d in [(3, 2), (5, 4), (17, 8), (77, 16), (1000, 16), (1024, 16), (600, 128), (333, 200), (130, 256),
+                 (1500, 16), (2048, 8), (1300, 32)]:
         out[f"normal-{n}x{d}"] = rng.normal(size=(n, d))
     e = rng.normal(size=(700, 128))
     e /= np.linalg.norm(e, axis=1, keepdims=True)
@@ -77,12 +78,24 @@ def gram_core_windows():
     return {k: v.astype(np.float32) for k, v in out.items()}
 
 
+_GRAM_CORE = {}
+
+
+def gram_core_cached():
+    if not _GRAM_CORE:
+        _GRAM_CORE.update(gram_core_windows())
+    return _GRAM_CORE
+
+
 @pytest.mark.parametrize("ms", [3, 4, 8, 9, 16])
-def test_core_gram_filter_bitwise(ms):
-    for name, x in gram_core_windows().items():
+@pytest.mark.parametrize("rsets", [0, 1, 2, 3])
+def test_core_gram_filter_bitwise(ms, rsets):
+    """rsets: flag bits 12..13, 16-row sets per wave of the Gram kernel (0:
+    by window size, four for 1025..2048 points; 1, 2, 3: one, two, four)."""
+    for name, x in gram_core_cached().items():
         if ms > len(x):
             continue
-        got, status = G.core(x, ms, "float32")
+        got, status = G.core(x, ms, "float32", rsets << 12)
         assert status == 0, name
         exp = oracle.core(oracle.distances(x, "float32"), ms, "float32")
         np.testing.assert_array_equal(got[0].view(np.uint32), exp.view(np.uint32), err_msg=name)
@@ -311,17 +324,34 @@ def test_points_path_edges_bitwise(precision, mcs, ms, eps, vpt, prefilter):
         np.testing.assert_array_equal(labels[0], t["labels"], err_msg=name)
 
 
-@pytest.mark.parametrize("index", range(len(WORKLOADS)), ids=[w[0] for w in WORKLOADS])
-def test_workload_edges_bitwise(index):
+def _edge_cases():
+    """Every workload window through the default plan, and the windows of
+    128 dimensions also through the materialised matrix with the MFMA Gram
+    filter forced (HDB_FLAG_MATRIX | HDB_FLAG_FILTER)."""
+    cases = [(i, 0) for i in range(len(WORKLOADS))]
+    cases += [(i, 2 | 128) for i, w in enumerate(WORKLOADS) if w[1].shape[1] == 128]
+    return cases
+
+
+EDGE_CASES = _edge_cases()
+
+
+@pytest.mark.parametrize("index,flags", EDGE_CASES, ids=[f"{WORKLOADS[i][0]}-f{f}" for i, f in EDGE_CASES])
+def test_workload_edges_bitwise(index, flags):
+    """Prim edges, tree intermediates and labels bitwise against the oracle's
+    stages, at every workload size (the oracle takes a few seconds for the
+    4096 x 32 float64 window)."""
     name, x, precision, mcs, ms, eps, _ = WORKLOADS[index]
-    if x.shape[0] * x.shape[0] * x.shape[1] > 2048 * 2048 * 32:
-        pytest.skip("oracle stage export too slow for this size")
-    labels, status, ex = G.cluster_export(x, mcs, ms, eps, precision)
+    labels, status, ex = G.cluster_export(x, mcs, ms, eps, precision, flags=flags)
     assert status == 0
     (u, v, w), t = oracle_window(x, mcs, ms, eps, precision)
     np.testing.assert_array_equal(ex[0]["edge_u"], u)
     np.testing.assert_array_equal(ex[0]["edge_v"], v)
-    np.testing.assert_array_equal(ex[0]["edge_w"], w)
+    np.testing.assert_array_equal(ex[0]["edge_w"].view(BITS[precision]), w.view(BITS[precision]))
+    for k in ("order", "left", "right", "sizes", "parent", "selected", "point_parent"):
+        np.testing.assert_array_equal(ex[0][k], t[k], err_msg=k)
+    for k in ("birth", "stability"):
+        np.testing.assert_array_equal(ex[0][k].view(BITS[precision]), t[k].view(BITS[precision]), err_msg=k)
     np.testing.assert_array_equal(labels[0], t["labels"])
 
 

@@ -9,7 +9,8 @@ Each oracle function is checked against an independent source:
              (reference tests/test_kernels.py:383-399, 782-803); MST total
              weight vs SciPy's minimum_spanning_tree
   argsort    np.argsort(kind="quicksort") with SIMD sorting disabled
-             (fixtures), 5 reference patterns x sizes 2..4096
+             (fixtures), 5 reference patterns x sizes 2..4096; the
+             depth-exhausted branch vs np.argsort(kind="heapsort") fixtures
   linkage    scikit-learn make_single_linkage (fp64, bitwise)
   condense   scikit-learn _condense_tree rows (fp64, bitwise)
   select     scikit-learn labels (exact) on every reference corpus and
@@ -27,6 +28,7 @@ from oracle import oracle
 STAGES = records("stages_f64")
 CORPUS = records("corpus")
 ARGSORT = records("argsort")
+HEAPSORT = records("heapsort")
 WORKLOADS = workload_windows()
 
 
@@ -42,6 +44,27 @@ def test_argsort_matches_numpy_scalar_quicksort(index):
     rec = ARGSORT[index]
     values = rec["w64"].astype(np.float32 if rec["is32"] else np.float64)
     np.testing.assert_array_equal(oracle.argsort(values), rec["order"])
+
+
+@pytest.mark.parametrize("index", range(len(HEAPSORT)), ids=[r["name"] for r in HEAPSORT])
+def test_depth_exhausted_argsort_matches_numpy_heapsort(index):
+    """A spent depth budget sends the whole span to NumPy's indirect heapsort
+    (reference hdbscan_gpu/_kernels.py:436-441): with the depth forced below
+    zero the oracle must give np.argsort(kind="heapsort")'s order (SIMD
+    sorting disabled; NumPy's heapsort has no SIMD variant anyway)."""
+    rec = HEAPSORT[index]
+    values = rec["w64"].astype(np.float32 if rec["is32"] else np.float64)
+    np.testing.assert_array_equal(oracle.argsort(values, depth=-1), rec["order"])
+
+
+def test_heapsort_fixtures_discriminate():
+    """The heapsort orders differ from the introsort's on most tie-heavy
+    arrays, so the test above cannot pass with the depth test ignored."""
+    differ = 0
+    for rec in HEAPSORT:
+        values = rec["w64"].astype(np.float32 if rec["is32"] else np.float64)
+        differ += not np.array_equal(oracle.argsort(values), rec["order"])
+    assert differ >= 40
 
 
 def test_planted_stable_sort_fails_argsort_fixtures():
