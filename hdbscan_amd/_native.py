@@ -17,7 +17,9 @@ FLAG_FILTER = 128
 FLAG_ONE_STEP = 65536
 EXPORTED = ("hdb_version", "hdb_last_error", "hdb_profile", "hdb_stage_count", "hdb_stage_name",
             "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_cluster_export", "hdb_distances",
-            "hdb_core_distances", "hdb_cluster_matrix", "hdb_ragged_workspace_bytes", "hdb_cluster_ragged")
+            "hdb_core_distances", "hdb_cluster_matrix", "hdb_ragged_workspace_bytes", "hdb_cluster_ragged",
+            "hdb_cluster_noalloc")
+NOALLOC_BYTES = 64 * 1024 * 1024
 
 _lib = None
 _P = ctypes.c_void_p
@@ -52,6 +54,9 @@ def lib():
         handle.hdb_cluster.restype = _I
         handle.hdb_cluster.argtypes = [_P, _P, _I64, _I64, _I64, _I, _I64, _I32, ctypes.c_double,
                                        _P, _P, ctypes.c_size_t, _P, _I]
+        handle.hdb_cluster_noalloc.restype = _I
+        handle.hdb_cluster_noalloc.argtypes = [_P, _P, _I64, _I64, _I64, _I, _I64, _I32, ctypes.c_double,
+                                               _P, _P, _I]
         handle.hdb_cluster_export.restype = _I
         handle.hdb_cluster_export.argtypes = [_P, _P, _I64, _I64, _I64, _I, _I64, _I32, ctypes.c_double, _I,
                                               _P, _P, ctypes.c_size_t, _P, _I, _P, _P]

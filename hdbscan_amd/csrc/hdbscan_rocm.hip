@@ -790,6 +790,30 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
   return err;
 }
 
+// Workspace of hdb_cluster_noalloc: one array per GPU in the library's own
+// device memory, held until the process ends.  In host code `scratch` names
+// a host-side stand-in; the device address comes from hipGetSymbolAddress
+// (scratch_base) and is the only form host code passes on.
+constexpr size_t kScratchBytes = (size_t)HDB_NOALLOC_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device, resolved once per device.
+int scratch_base(unsigned char** out) {
+  static std::mutex mu;
+  static void* base[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail("no current HIP device");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!base[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    base[dev] = p;
+  }
+  *out = static_cast<unsigned char*>(base[dev]);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -835,6 +859,31 @@ int hdb_cluster(void* stream, const void* x, int64_t batch, int64_t n, int64_t d
   if (workspace_bytes < P.total || (P.total && !workspace)) return fail("workspace too small");
   hipStream_t s = static_cast<hipStream_t>(stream);
   unsigned char* ws = static_cast<unsigned char*>(workspace);
+  const bool unit = min_samples <= 2;
+  const Launch L = dense_launch(batch, n);
+  g_timers.count = 0;
+  if (dtype == HDB_F32)
+    return run_cluster<float>(s, P, static_cast<const float*>(x), nullptr, nullptr, L, d, min_cluster_size,
+                              min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
+  return run_cluster<double>(s, P, static_cast<const double*>(x), nullptr, nullptr, L, d, min_cluster_size,
+                             min_samples, unit, epsilon, -100000, labels, ws, status, nullptr, nullptr);
+}
+
+int hdb_cluster_noalloc(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,
+                        int64_t min_cluster_size, int32_t min_samples, double epsilon, int32_t* labels,
+                        int32_t* status, int flags) {
+  g_error.clear();
+  if (!x || !labels || !status) return fail("null pointer argument");
+  if (min_cluster_size < 2) return fail("min_cluster_size must be >= 2");
+  Plan P;
+  if (make_plan(batch, n, d, dtype, min_samples, flags, &P)) return -1;
+  if (P.total > kScratchBytes)
+    return fail("hdb_cluster_noalloc: this call needs " + std::to_string(P.total) +
+                " bytes of workspace, above its fixed " + std::to_string(kScratchBytes) +
+                "-byte scratch; split the batch or call hdb_cluster with a workspace");
+  unsigned char* ws = nullptr;
+  if (int r = scratch_base(&ws)) return r;
+  hipStream_t s = static_cast<hipStream_t>(stream);
   const bool unit = min_samples <= 2;
   const Launch L = dense_launch(batch, n);
   g_timers.count = 0;

@@ -92,6 +92,24 @@ int hdb_cluster(void *stream, const void *x, int64_t batch, int64_t n, int64_t d
                 double epsilon, int32_t *labels, void *workspace,
                 size_t workspace_bytes, int32_t *status, int flags);
 
+/* hdb_cluster without a workspace argument: it allocates nothing.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its workspace is a fixed array of
+ * HDB_NOALLOC_BYTES in the library's device memory, one per GPU, held until
+ * the process ends.  A call whose hdb_workspace_bytes(batch, n, d, dtype,
+ * min_samples, flags) exceeds HDB_NOALLOC_BYTES fails with an error (B=256
+ * windows of 1024 points in 16 float32 dimensions fit).  Every call on a GPU
+ * shares that GPU's array: calls enqueued on one stream run one after the
+ * other, but calls that overlap on one GPU (different streams, or threads
+ * without ordering) race and give undefined labels.  Arguments, status and
+ * results as hdb_cluster. */
+#define HDB_NOALLOC_BYTES (64u * 1024u * 1024u)
+int hdb_cluster_noalloc(void *stream, const void *x, int64_t batch, int64_t n,
+                        int64_t d, int dtype, int64_t min_cluster_size,
+                        int32_t min_samples, double epsilon, int32_t *labels,
+                        int32_t *status, int flags);
+
 /* hdb_cluster with every intermediate exported (layout of hdb_cluster_matrix
  * below) and an optional forced introsort depth: the white-box view of the
  * production path.  Replaces the reference's diagnostic kernels that export

@@ -94,3 +94,13 @@ def test_product_path_never_references_the_oracle():
                 if f.endswith((".py", ".hip", ".h", ".cpp")):
                     text = open(os.path.join(dirpath, f)).read()
                     assert "oracle" not in text.lower().replace("oracle (", ""), os.path.join(dirpath, f)
+
+
+def test_noalloc_rejects_calls_above_its_scratch_before_any_device_work():
+    lib = _native.lib()
+    p = ctypes.c_void_p(1)
+    assert lib.hdb_workspace_bytes(512, 1024, 16, _native.F32, 5, 0) > _native.NOALLOC_BYTES
+    rc = lib.hdb_cluster_noalloc(None, p, 512, 1024, 16, _native.F32, 5, 5, 0.0, p, p, 0)
+    assert rc != 0 and "scratch" in lib.hdb_last_error().decode()
+    rc = lib.hdb_cluster_noalloc(None, None, 1, 8, 2, _native.F32, 2, 2, 0.0, p, p, 0)
+    assert rc != 0 and "null" in lib.hdb_last_error().decode()
