@@ -108,6 +108,32 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
   if (__any(nonfinite) && (tid & 63) == 0) atomicOr(status, 2);
 }
 
+// Issue-rate probe for the exact distance's instruction mix: per coordinate
+// one subtraction, one multiplication and one addition, no FMA (built with
+// -ffp-contract=off) and no packed fp32.  Each thread runs 8 independent
+// chains t = t + (a - t) * (a - t) for `iters` rounds: 24 operations per
+// round; the result is stored so nothing is removed.  Values may become
+// infinite or NaN: the issue rate does not depend on them.
+__global__ __launch_bounds__(256) void valu_probe_kernel(int iters, float seed, float* out) {
+  float a[8], t[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    a[i] = seed + (float)threadIdx.x * 1e-3f + (float)i;
+    t[i] = (float)i * 1e-2f;
+  }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const float dl = a[i] - t[i];
+      t[i] = t[i] + dl * dl;
+    }
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s = s + t[i];
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 // Upper bound of b*b (see square_bound in hdb_prim.h).
 __device__ __forceinline__ float sq_bound(float b) {
   const float p = b * b;

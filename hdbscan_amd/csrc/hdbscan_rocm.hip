@@ -842,6 +842,24 @@ float hdb_stage_ms(int i) {
 
 const char* hdb_last_error(void) { return g_error.c_str(); }
 
+int hdb_probe_valu(void* stream, int blocks, int iters, float* out, float* ms) {
+  g_error.clear();
+  if (!out || !ms || blocks < 1 || iters < 1) return fail("invalid probe arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("cannot create events");
+  valu_probe_kernel<<<(unsigned)blocks, 256, 0, s>>>(iters, 1.0f, out);  // warm-up
+  hipEventRecord(e0, s);
+  valu_probe_kernel<<<(unsigned)blocks, 256, 0, s>>>(iters, 1.0f, out);
+  hipEventRecord(e1, s);
+  int rc = check_launch("valu_probe_kernel");
+  if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail("probe synchronisation failed");
+  if (!rc && hipEventElapsedTime(ms, e0, e1) != hipSuccess) rc = fail("probe timing failed");
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
+}
+
 size_t hdb_workspace_bytes(int64_t batch, int64_t n, int64_t d, int dtype, int32_t min_samples, int flags) {
   Plan P;
   if (make_plan(batch, n, d, dtype, min_samples, flags, &P)) return 0;

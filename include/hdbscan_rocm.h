@@ -77,6 +77,15 @@ int hdb_stage_count(void);
 const char *hdb_stage_name(int i);
 float hdb_stage_ms(int i);
 
+/* Measured VALU issue rate of the exact distance's instruction mix (sub,
+ * mul, add; no FMA, no packed fp32), the attainable ceiling of the
+ * exact-distance kernels.  No counterpart in the reference (a measurement
+ * diagnostic, like the stage timers).  Runs `blocks` workgroups of 256
+ * threads, each thread 24 * iters operations, on `stream` (one untimed
+ * warm-up launch); out: a device buffer of blocks * 256 floats; *ms: the
+ * timed launch's duration.  Synchronises the stream. */
+int hdb_probe_valu(void *stream, int blocks, int iters, float *out, float *ms);
+
 /* Bytes of device workspace hdb_cluster needs for these arguments.
  * Replaces the reference's global-workspace sizing
  * (reference hdbscan_gpu/_cuda.py:62-64, 119-121). */
