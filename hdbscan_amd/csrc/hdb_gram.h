@@ -138,11 +138,13 @@ constexpr float kGramEh = 7.7037197775489434e-34f;  // 2^-110
 // Per point p of the batch: pa[p] = bf16 (hi(x) | lo(x)), pb[p] = (hi(-2x) |
 // lo(-2x)), each DK wide (zero padded); nb[p] = |x|^2 (1 + c) + eh,
 // nb[P + p] = |x|^2 (1 - c) - eh, nb[2P + p] = |x|^2.
+// Non-finite coordinates OR HDB_STATUS_NONFINITE (2) into *status.
 template <int DK>
 __global__ __launch_bounds__(256) void gram_prep_kernel(const float* x, int64_t points, int d, uint16_t* pa,
-                                                        uint16_t* pb, float* nb) {
+                                                        uint16_t* pb, float* nb, int32_t* status) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= points) return;
+  int bad = 0;
+  if (p < points) {
   const float* xp = x + (size_t)p * d;
   uint16_t* ap = pa + (size_t)p * 2 * DK;
   uint16_t* bp = pb + (size_t)p * 2 * DK;
@@ -152,6 +154,8 @@ __global__ __launch_bounds__(256) void gram_prep_kernel(const float* x, int64_t 
     float v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = k0 + e < d ? xp[k0 + e] : 0.f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bad |= !isfinite(v[e]);
     gram_s16x8 ah, al, bh, bl;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
@@ -172,6 +176,8 @@ __global__ __launch_bounds__(256) void gram_prep_kernel(const float* x, int64_t 
   nb[p] = fmaf(sq, c, sq) + kGramEh;
   nb[points + p] = fmaf(sq, -c, sq) - kGramEh;
   nb[2 * points + p] = sq;
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 2);
 }
 
 // LDS bytes of core_gram_kernel<.., DK, RSET> for a window of n points.
@@ -403,7 +409,7 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
 // instruction); the per-point arithmetic is that of gram_prep_kernel.
 template <int DK>
 __global__ __launch_bounds__(256) void gram_prep_lds_kernel(const float* x, int64_t points, int d, uint16_t* pa,
-                                                            uint16_t* pb, float* nb) {
+                                                            uint16_t* pb, float* nb, int32_t* status) {
   static_assert(DK <= 16, "LDS staging sized for DK <= 16");
   __shared__ __align__(16) float xin[256 * DK];
   __shared__ __align__(16) uint16_t oa[256 * 2 * DK];
@@ -419,6 +425,7 @@ __global__ __launch_bounds__(256) void gram_prep_lds_kernel(const float* x, int6
     for (int k = t; k < total; k += 256) xin[k] = xb[k];
   }
   __syncthreads();
+  int bad = 0;
   if (t < cnt) {
     const float c = gram_c(DK);
     float sq = 0.f;
@@ -429,6 +436,8 @@ __global__ __launch_bounds__(256) void gram_prep_lds_kernel(const float* x, int6
       float v[8];
 #pragma unroll
       for (int e = 0; e < 8; ++e) v[e] = k0 + e < d ? xin[t * d + k0 + e] : 0.f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) bad |= !isfinite(v[e]);
       gram_s16x8 ah, al, bh, bl;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -461,6 +470,7 @@ __global__ __launch_bounds__(256) void gram_prep_lds_kernel(const float* x, int6
     ga[k] = reinterpret_cast<const uint4*>(oa)[k];
     gb[k] = reinterpret_cast<const uint4*>(ob)[k];
   }
+  if (__any(bad) && (t & 63) == 0) atomicOr(status, 2);
 }
 
 // Exact coordinate-order squared distance of rows xi and xj (d floats), 16

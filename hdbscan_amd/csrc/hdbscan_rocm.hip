@@ -25,6 +25,7 @@
 #include "../../include/hdbscan_rocm.h"
 #include "hdb_common.h"
 #include "hdb_gram.h"
+#include "hdb_gramgraph.h"
 #include "hdb_kernels.h"
 #include "hdb_prim.h"
 
@@ -126,6 +127,8 @@ struct Plan {
   size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0, off_matrix = 0;
   size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, off_pack = 0, total = 0;
   int repack = 0;  // Prim repacking interval (points path)
+  bool gram_graph = false;  // matrix path through the MFMA Gram matrix (hdb_gramgraph.h)
+  size_t off_alt = 0;
 };
 
 // One launch of the pipeline: the kernels' window descriptor, the grid's
@@ -265,7 +268,18 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
     P.vpt = vpt = 8;
     P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
   }
-  set_gram(P, n, d, ms, lds_limit);
+  // float32 windows on the matrix path take the MFMA Gram form of the graph
+  // stage unless the exact matrix is asked for (plain, exact graph, a given
+  // matrix, or the exact-matrix Prim variants).
+  P.gram_graph = P.matrix && rs == 4 && d <= 256 && ms <= 64 &&
+                 !(flags & (HDB_FLAG_PLAIN | HDB_FLAG_EXACT_GRAPH | HDB_FLAG_MATRIX_INPUT | HDB_FLAG_ONE_STEP)) &&
+                 want_vpt != 8;
+  if (P.gram_graph) {
+    P.gram_dk = d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
+    if (P.vpt > 4) return fail("window too large for the vertex layout");
+  } else {
+    set_gram(P, n, d, ms, lds_limit);
+  }
   const uint32_t core_bytes = P.core ? (uint32_t)(n * rs) : 0u;
   if (!P.matrix) {
     // points (optional) | core distances (when needed) | squared norms
@@ -275,13 +289,14 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
     // Without LDS points, one byte per vertex records the wave holding it.
     P.prim_lds = (uint32_t)((P.xlds ? xs : 0) + core_bytes + norms + (P.xlds ? 0 : align16((uint32_t)n)));
   } else {
-    P.prim_lds = core_bytes;
+    P.prim_lds = P.gram_graph ? (uint32_t)(2 * n * 4) : core_bytes;
   }
   const TreeLayout L = tree_layout((int)n, rs);
   P.tree_lds = !(flags & HDB_FLAG_GLOBAL_ARENA) && L.total + 1024 <= lds_limit;
   P.tree_lds_bytes = P.tree_lds ? L.total : 0u;
   P.arena_stride = P.tree_lds ? 0 : align256(L.total);
   P.need_matrix = !(flags & HDB_FLAG_MATRIX_INPUT) && (P.matrix || (P.core && !P.core_topk));
+  const bool pieces_needed = P.core_gram || P.gram_graph;
   size_t o = 0;
   const size_t bn = (size_t)batch * n;
   P.off_eu = o; o = align256(o + bn * 4);
@@ -295,10 +310,11 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.off_matrix = o; o = align256(o + (P.need_matrix ? bn * (size_t)n * rs : 0));
   P.off_cand = o; o = align256(o + (P.core_gram ? bn * kGramCap * 2 : 0));
   P.off_ncand = o; o = align256(o + (P.core_gram ? bn * 4 : 0));
-  const size_t pieces = P.core_gram ? bn * 2 * (size_t)P.gram_dk * 2 : 0;
+  const size_t pieces = pieces_needed ? bn * 2 * (size_t)P.gram_dk * 2 : 0;
   P.off_pa = o; o = align256(o + pieces);
   P.off_pb = o; o = align256(o + pieces);
-  P.off_nb = o; o = align256(o + (P.core_gram ? 3 * bn * 4 : 0));
+  P.off_nb = o; o = align256(o + (pieces_needed ? 3 * bn * 4 : 0));
+  P.off_alt = o; o = align256(o + (P.gram_graph ? bn * 4 : 0));
   // Prim repacking of the points path: flag bits 14..15 (0: every 128
   // steps, 1: never, 2: every 32, 3: every 64).  128 measured best: equal
   // or faster than 64 at N=1024 and 4 % faster on ragged batches, whose
@@ -382,18 +398,102 @@ GramBuffers gram_buffers(const Plan& P, unsigned char* ws) {
 }
 
 // bf16 pieces and norm terms of every point of the batch.
-int launch_gram_prep(hipStream_t s, const Plan& P, const float* x, int64_t points, int64_t d, unsigned char* ws) {
+int launch_gram_prep(hipStream_t s, const Plan& P, const float* x, int64_t points, int64_t d, unsigned char* ws,
+                     int32_t* status) {
   const GramBuffers g = gram_buffers(P, ws);
   const unsigned blocks = (unsigned)((points + 255) / 256);
   switch (P.gram_dk) {
-    case 8: gram_prep_lds_kernel<8><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
-    case 16: gram_prep_lds_kernel<16><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
-    case 32: gram_prep_kernel<32><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
-    case 64: gram_prep_kernel<64><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
-    case 128: gram_prep_kernel<128><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
-    default: gram_prep_kernel<256><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb); break;
+    case 8: gram_prep_lds_kernel<8><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb, status); break;
+    case 16: gram_prep_lds_kernel<16><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb, status); break;
+    case 32: gram_prep_kernel<32><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb, status); break;
+    case 64: gram_prep_kernel<64><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb, status); break;
+    case 128: gram_prep_kernel<128><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb, status); break;
+    default: gram_prep_kernel<256><<<blocks, 256, 0, s>>>(x, points, (int)d, g.pa, g.pb, g.nb, status); break;
   }
   return check_launch("gram_prep_kernel");
+}
+
+// The MFMA Gram form of the graph stage (hdb_gramgraph.h): G, core
+// distances from G, interval-key Prim, exact weights of the open edges.
+template <int DK>
+int launch_gram_matrix(hipStream_t s, const float* x, int64_t batch, int64_t n, int64_t d, const GramBuffers& g,
+                       int64_t points, float* G, int32_t* status) {
+  auto fn = gram_matrix_kernel<DK>;
+  if (int r = allow_lds((const void*)fn, kGramMatrixLds)) return r;
+  const unsigned tps = (unsigned)((n + kGramTile - 1) / kGramTile);
+  fn<<<dim3(tps * tps, (unsigned)batch), 256, kGramMatrixLds, s>>>(x, (int)n, (int)d, g.pa, g.pb, g.nb + 2 * points,
+                                                                     G, status);
+  return check_launch("gram_matrix_kernel");
+}
+
+template <int VPT>
+int launch_prim_gram(hipStream_t s, const Plan& P, int64_t batch, const GramPrimArgs& a) {
+  auto fn = prim_gram_kernel<VPT>;
+  if (int r = allow_lds((const void*)fn, P.prim_lds)) return r;
+  fn<<<(unsigned)batch, P.threads, P.prim_lds, s>>>(a);
+  return check_launch("prim_gram_kernel");
+}
+
+int run_gram_graph(hipStream_t s, const Plan& P, const float* x, const Launch& L, int64_t d, int ms, bool unit,
+                   unsigned char* ws, int32_t* status) {
+  const int64_t batch = L.count, n = L.nmax, points = L.points;
+  const GramBuffers g = gram_buffers(P, ws);
+  float* G = reinterpret_cast<float*>(ws + P.off_matrix);
+  float* core = reinterpret_cast<float*>(ws + P.off_core);
+  if (int r = launch_gram_prep(s, P, x, points, d, ws, status)) return r;
+  stage_mark(s, "gram_prep_kernel");
+  int r = 0;
+  switch (P.gram_dk) {
+    case 8: r = launch_gram_matrix<8>(s, x, batch, n, d, g, points, G, status); break;
+    case 16: r = launch_gram_matrix<16>(s, x, batch, n, d, g, points, G, status); break;
+    case 32: r = launch_gram_matrix<32>(s, x, batch, n, d, g, points, G, status); break;
+    case 64: r = launch_gram_matrix<64>(s, x, batch, n, d, g, points, G, status); break;
+    case 128: r = launch_gram_matrix<128>(s, x, batch, n, d, g, points, G, status); break;
+    default: r = launch_gram_matrix<256>(s, x, batch, n, d, g, points, G, status); break;
+  }
+  if (r) return r;
+  stage_mark(s, "gram_matrix_kernel");
+  const float c2 = gram_c2(P.gram_dk);
+  const float* norms = g.nb + 2 * points;
+  if (!unit) {
+    const unsigned blocks = (unsigned)((points + 3) / 4);
+    if (ms <= 8)
+      core_from_gram_kernel<8><<<blocks, 256, 0, s>>>(G, x, norms, points, (int)n, (int)d, ms, c2, core, status);
+    else if (ms <= 16)
+      core_from_gram_kernel<16><<<blocks, 256, 0, s>>>(G, x, norms, points, (int)n, (int)d, ms, c2, core, status);
+    else if (ms <= 32)
+      core_from_gram_kernel<32><<<blocks, 256, 0, s>>>(G, x, norms, points, (int)n, (int)d, ms, c2, core, status);
+    else
+      core_from_gram_kernel<64><<<blocks, 256, 0, s>>>(G, x, norms, points, (int)n, (int)d, ms, c2, core, status);
+    if (int e = check_launch("core_from_gram_kernel")) return e;
+    stage_mark(s, "core_from_gram_kernel");
+  }
+  GramPrimArgs a;
+  a.g = G;
+  a.x = x;
+  a.core = unit ? nullptr : core;
+  a.norms = norms;
+  a.n = (int)n;
+  a.d = (int)d;
+  a.unit_core = unit ? 1 : 0;
+  a.c2 = c2;
+  a.edge_u = reinterpret_cast<int32_t*>(ws + P.off_eu);
+  a.edge_v = reinterpret_cast<int32_t*>(ws + P.off_ev);
+  a.edge_w = reinterpret_cast<float*>(ws + P.off_ew);
+  a.edge_alt = reinterpret_cast<int32_t*>(ws + P.off_alt);
+  a.wflag = reinterpret_cast<int32_t*>(ws + P.off_flag);
+  a.status = status;
+  switch (P.vpt) {
+    case 1: r = launch_prim_gram<1>(s, P, batch, a); break;
+    case 2: r = launch_prim_gram<2>(s, P, batch, a); break;
+    default: r = launch_prim_gram<4>(s, P, batch, a); break;
+  }
+  if (r) return r;
+  stage_mark(s, "prim_kernel");
+  edge_fix_kernel<<<dim3((unsigned)((n - 1 + 255) / 256), (unsigned)batch), 256, 0, s>>>(a);
+  if (int e = check_launch("edge_fix_kernel")) return e;
+  stage_mark(s, "edge_fix_kernel");
+  return 0;
 }
 
 // Candidates (MFMA bounds), then the exact selection (after the prep, which
@@ -403,7 +503,7 @@ int launch_gram_prep(hipStream_t s, const Plan& P, const float* x, int64_t point
 int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist, const Launch& L, int64_t d,
                 int ms, float* core, unsigned char* ws, int32_t* status, bool prep) {
   if (prep) {
-    if (int r = launch_gram_prep(s, P, x, L.points, d, ws)) return r;
+    if (int r = launch_gram_prep(s, P, x, L.points, d, ws, status)) return r;
     stage_mark(s, "gram_prep_kernel");
   }
   const GramBuffers g = gram_buffers(P, ws);
@@ -547,7 +647,15 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   const Real* dist = given_dist;
   const Real* core_in = given_core;
   stage_mark(s, "start");
-  if (!given_dist) {
+  bool prim_done = false;
+  if constexpr (sizeof(Real) == 4) {
+    if (P.gram_graph && !given_dist) {
+      if (L.win.off) return fail("internal: ragged batch on the matrix path");
+      if (int r = run_gram_graph(s, P, x, L, d, ms, unit, ws, status)) return r;
+      prim_done = true;
+    }
+  }
+  if (!given_dist && !prim_done) {
     bool matrix_ready = false;
     if (P.matrix) {
       if (L.win.off) return fail("internal: ragged batch on the matrix path");
@@ -578,8 +686,10 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   pa.edge_w = ew;
   pa.wflag = wflag;
   pa.status = status;
-  if (int r = launch_prim<Real>(s, P, L.count, pa)) return r;
-  stage_mark(s, "prim_kernel");
+  if (!prim_done) {
+    if (int r = launch_prim<Real>(s, P, L.count, pa)) return r;
+    stage_mark(s, "prim_kernel");
+  }
   TreeArgs ta;
   ta.edge_u = eu;
   ta.edge_v = ev;
@@ -748,7 +858,7 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
     if (R.gram) {
       int k0 = 0;
       while (!R.used[k0]) ++k0;
-      err = launch_gram_prep(s, R.plan[k0], x, R.points, d, ws);
+      err = launch_gram_prep(s, R.plan[k0], x, R.points, d, ws, status);
     }
   }
   if (!err && hipEventRecord(S->fork, s) != hipSuccess) err = fail("event record failed");
@@ -957,7 +1067,8 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
   // min_samples > 64) and the Gram candidate lists (3 <= min_samples <= 16).
   Plan P;
   const int filter = (flags & HDB_FLAG_PLAIN) ? 0 : HDB_FLAG_FILTER;
-  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX | filter, &P)) return -1;
+  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_EXACT_GRAPH | filter, &P))
+    return -1;
   P.core = true;
   P.core_topk = min_samples <= 64;
   P.kmax = min_samples <= 8 ? 8 : min_samples <= 16 ? 16 : min_samples <= 32 ? 32 : 64;

@@ -7,7 +7,7 @@ import pytest
 import gpu_helpers as G
 from golden_io import records, same_partition, workload_windows
 from oracle import oracle
-from hdbscan_amd._native import FLAG_ONE_STEP
+from hdbscan_amd._native import FLAG_EXACT_GRAPH, FLAG_ONE_STEP
 
 pytestmark = pytest.mark.gpu
 
@@ -357,11 +357,16 @@ def test_workload_edges_bitwise(index, flags):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("ms", [3, 5, 9, 17, 40, 70])
-@pytest.mark.parametrize("flags", [2 | 64, 2, 2 | 128])
+@pytest.mark.parametrize("flags", [2 | 64, 2, 2 | 128, 2 | 1 << 8, 2 | 4 << 8, 2 | FLAG_EXACT_GRAPH,
+                                   2 | FLAG_EXACT_GRAPH | 128])
 def test_matrix_path_core_selection_edges(precision, ms, flags):
-    """Core distances selected from the materialised matrix (register lists
-    up to min_samples 64, bitwise search beyond), or the Gram filter with
-    matrix lookups: Prim edge weights (which contain them) bitwise."""
+    """float32 matrix path: the MFMA Gram matrix with interval-key Prim and
+    core distances from its bounds (flags 2, any vertices per thread; the
+    stress windows put separations far inside the bound, so most choices
+    are resolved exactly), or the exact matrix (HDB_FLAG_EXACT_GRAPH, plain)
+    with core distances selected from it or through the Gram filter; float64
+    always takes the exact matrix.  Prim edge weights (which contain the core
+    distances) bitwise."""
     for name, x in prefilter_windows(precision).items():
         labels, status, ex = G.cluster_export(x, 5, ms, 0.0, precision, flags=flags)
         assert status == 0, name
