@@ -51,12 +51,71 @@ __device__ __forceinline__ float gram_err(float c2, float ni, float nj) {
   return (ni <= kGramNormLimit && nj <= kGramNormLimit) ? c2 * (ni + nj) + kGramEabs : 0.f;
 }
 
+// Squared norms of the batch's points (an FMA chain; any order stays within
+// the bound's gamma_D term) and the input check (HDB_STATUS_NONFINITE).
+__global__ __launch_bounds__(256) void gram_norm_kernel(const float* x, int64_t points, int d, float* norms,
+                                                        int32_t* status) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int bad = 0;
+  if (p < points) {
+    const float* xp = x + (size_t)p * d;
+    float sq = 0.f;
+    for (int k = 0; k < d; ++k) {
+      const float v = xp[k];
+      bad |= !isfinite(v);
+      sq = fmaf(v, v, sq);
+    }
+    norms[p] = sq;
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 2);
+}
+
+// MFMA operand arrays of the Gram matrix, in the order the matrix cores take
+// them: for window w, 16-point block k (points past the window are zero) and
+// K-step q (32 bf16), entry (((w * nblk + k) * NQ + q) * 64 + lane) holds the
+// 8 bf16 operand values of lane `lane`: point 16 k + (lane & 15), K indices
+// 32 q + 8 (lane >> 4) ..., of A = [hi(x) | hi(x) | lo(x)] and of
+// B = [hi(-2x) | lo(-2x) | hi(-2x)] (zero past 3 DK).  A wave then loads an
+// operand block as one contiguous kilobyte.
+template <int DK>
+__global__ __launch_bounds__(256) void gram_operand_kernel(const float* x, int n, int d, int64_t entries, int nblk,
+                                                           gram_s16x8* aop, gram_s16x8* bop) {
+  constexpr int NQ = GramShape<DK, 1>::NQ;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= entries) return;
+  const int lane = (int)(e & 63);
+  const int64_t kq = e >> 6;
+  const int q = (int)(kq % NQ);
+  const int64_t kb = kq / NQ;
+  const int64_t w = kb / nblk;
+  const int i = (int)(kb % nblk) * 16 + (lane & 15);
+  const int kk = 32 * q + 8 * (lane >> 4);
+  const int seg = kk / DK, m = kk - seg * DK;
+  gram_s16x8 av = {}, bv = {};
+  if (seg < 3 && i < n) {
+    const float* xp = x + ((size_t)w * n + i) * d;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const float v = m + t < d ? xp[m + t] : 0.f;
+      const uint32_t h = bf16_bits(v);
+      const uint32_t l = bf16_bits(v - bf16_value(h));
+      const float v2 = -2.f * v;
+      const uint32_t h2 = bf16_bits(v2);
+      const uint32_t l2 = bf16_bits(v2 - bf16_value(h2));
+      av[t] = (short)(seg == 2 ? l : h);
+      bv[t] = (short)(seg == 1 ? l2 : h2);
+    }
+  }
+  aop[e] = av;
+  bop[e] = bv;
+}
+
 // One workgroup computes a kGramTile x kGramTile block of G[b] (256 threads,
 // each wave a 64 x 64 quarter as 4 x 4 MFMA blocks of 16 x 16).  The
 // diagonal is stored as +0 (s_ii = 0).  Grid: (tiles per side ^ 2, batch).
 template <int DK>
-__global__ __launch_bounds__(256) void gram_matrix_kernel(const float* x, int n, int d, const uint16_t* pa,
-                                                          const uint16_t* pb, const float* norms, float* G,
+__global__ __launch_bounds__(256) void gram_matrix_kernel(const float* x, int n, int d, const gram_s16x8* aop,
+                                                          const gram_s16x8* bop, const float* norms, float* G,
                                                           int32_t* status) {
   using Sh = GramShape<DK, 1>;
   constexpr int NQ = Sh::NQ;
@@ -101,24 +160,25 @@ __global__ __launch_bounds__(256) void gram_matrix_kernel(const float* x, int n,
       for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = gram_f32x4{nr[0] + nc[cb], nr[1] + nc[cb], nr[2] + nc[cb], nr[3] + nc[cb]};
     }
   }
-  const uint16_t* arow[4];
-  const uint16_t* bcol[4];
+  // Operand blocks of the wave's 4 row blocks and 4 column blocks (blocks
+  // past the window are clamped to its last one; their results are not
+  // stored).
+  const int nblk = (n + 15) / 16;
+  const gram_s16x8* arow[4];
+  const gram_s16x8* bcol[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    arow[q] = pa + (size_t)(base + min(r0 + wr + 16 * q + l16, n - 1)) * 2 * DK;
-    bcol[q] = pb + (size_t)(base + min(c0 + wc + 16 * q + l16, n - 1)) * 2 * DK;
+    const int ka = min((r0 + wr) / 16 + q, nblk - 1), kb = min((c0 + wc) / 16 + q, nblk - 1);
+    arow[q] = aop + ((size_t)b * nblk + ka) * NQ * 64 + lane;
+    bcol[q] = bop + ((size_t)b * nblk + kb) * NQ * 64 + lane;
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    // Operand A of a row block: [hi(x) | hi(x) | lo(x)]; B of a column
-    // block: [hi(-2x) | lo(-2x) | hi(-2x)] (zero past 3 DK).
-    const int sa = gram_src<DK>(32 * q + 8 * grp, 2);
-    const int sb = gram_src<DK>(32 * q + 8 * grp, 1);
     gram_bf16x8 av[4], bv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      av[k] = sa < 0 ? gram_bf16x8{} : *reinterpret_cast<const gram_bf16x8*>(arow[k] + sa);
-      bv[k] = sb < 0 ? gram_bf16x8{} : *reinterpret_cast<const gram_bf16x8*>(bcol[k] + sb);
+      av[k] = __builtin_bit_cast(gram_bf16x8, arow[k][q * 64]);
+      bv[k] = __builtin_bit_cast(gram_bf16x8, bcol[k][q * 64]);
     }
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
@@ -161,15 +221,15 @@ __global__ __launch_bounds__(256) void gram_matrix_kernel(const float* x, int n,
 constexpr int kCoreCands = 128;
 
 // The kth smallest (1-based) of the union of the 64 lanes' ascending lists
-// (equal heads pop together); uniform trip count.
+// of non-negative values (equal heads pop together; the minimum is an
+// unsigned minimum of the bit patterns over DPP levels).  Uniform trip
+// count: all lanes active.
 template <int K>
 __device__ __forceinline__ float wave_kth(float* lst, int kth) {
   int cnt = 0;
   float ans = INFINITY;
   for (int it = 0; it < kth && cnt < kth; ++it) {
-    float m = lst[0];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) m = fminf(m, __shfl_xor(m, o));
+    const float m = __uint_as_float(wave_umin32(__float_as_uint(lst[0])));
     const bool eq = lst[0] == m;
     cnt += __popcll(__ballot(eq));
     ans = m;
@@ -180,6 +240,54 @@ __device__ __forceinline__ float wave_kth(float* lst, int kth) {
     }
   }
   return ans;
+}
+
+// Exact coordinate-order squared distances from row xj to up to three rows
+// xa[0..P-1] (d <= 256 floats), evaluated by the whole wave: lane l loads
+// coordinates l, l + 64, ... of every row (all loads issued together), squares
+// the differences, and each pair's terms are then added in ascending
+// coordinate order from the lanes' registers, exactly as the reference's loop
+// adds them (t = 0; t = t + dl*dl); the P sums advance together.  Terms past
+// d are +0 and leave a sum unchanged.  Arguments uniform; all lanes active.
+template <int P>
+__device__ __forceinline__ void wave_exact_sq(const float* xj, const float* const* xa, int d, float* out) {
+  const int lane = threadIdx.x & 63;
+  float vj[4], va[P][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int k = lane + 64 * m;
+    const bool in = k < d;
+    vj[m] = in ? xj[k] : 0.f;
+#pragma unroll
+    for (int p = 0; p < P; ++p) va[p][m] = in ? xa[p][k] : 0.f;
+  }
+  float sq[P][4], t[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    t[p] = 0.f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const float dl = va[p][m] - vj[m];
+      sq[p][m] = dl * dl;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    if (64 * m >= d) break;
+#pragma unroll
+    for (int l = 0; l < 64; ++l)
+#pragma unroll
+      for (int p = 0; p < P; ++p) t[p] = t[p] + __uint_as_float(read_lane(__float_as_uint(sq[p][m]), l));
+  }
+#pragma unroll
+  for (int p = 0; p < P; ++p) out[p] = t[p];
+}
+
+__device__ __forceinline__ float wave_exact_sq1(const float* xa, const float* xb, int d) {
+  float o[1];
+  const float* rows[1] = {xa};
+  wave_exact_sq<1>(xb, rows, d, o);
+  return o[0];
 }
 
 template <int KMAX>
@@ -195,28 +303,55 @@ __global__ __launch_bounds__(256) void core_from_gram_kernel(const float* G, con
   const float* gi = G + (size_t)r * n;
   const float ni = norms[r];
   const float* xi = x + (size_t)r * d;
-  float lst[KMAX];
+  // Pass A: U = the ms-th smallest upper bound of the row (>= the exact
+  // statistic t), and Emax = the row's largest bound term.  The ms columns of
+  // the exact ms smallest values have ub - s <= 2 E_j plus the rounding of
+  // g +- E (below 2u |g| <= 0.01 E_j for every c2 used here), so
+  // t >= U - 2.01 Emax =: L, taken a few ulps lower still (its own rounding).
+  // Eight loads in flight.
+  float lub[KMAX];
 #pragma unroll
-  for (int q = 0; q < KMAX; ++q) lst[q] = INFINITY;
-  for (int j = lane; j < n; j += 64) {
-    const float g = gi[j];
-    const float ub = j == i ? 0.f : g + gram_err(c2, ni, norms[base + j]);
-    if (ub < lst[KMAX - 1]) {
+  for (int q = 0; q < KMAX; ++q) lub[q] = INFINITY;
+  float emax = 0.f;
+  for (int j0 = 0; j0 < n; j0 += 8 * 64) {
+    float g[8], nj[8];
 #pragma unroll
-      for (int q = KMAX - 1; q > 0; --q) lst[q] = __builtin_amdgcn_fmed3f(lst[q - 1], lst[q], ub);
-      lst[0] = ub < lst[0] ? ub : lst[0];
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + u * 64 + lane;
+      g[u] = j < n ? gi[j] : INFINITY;
+      nj[u] = j < n ? norms[base + j] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + u * 64 + lane;
+      const float e = gram_err(c2, ni, nj[u]);
+      emax = fmaxf(emax, e);
+      const float ub = j == i ? 0.f : fmaxf(g[u] + e, 0.f);  // +inf past the row
+#pragma unroll
+      for (int q = KMAX - 1; q > 0; --q) lub[q] = __builtin_amdgcn_fmed3f(lub[q - 1], lub[q], ub);
+      lub[0] = fminf(lub[0], ub);
     }
   }
-  const float U = wave_kth<KMAX>(lst, ms);
-  int cnt = 0;
+  const float U = wave_kth<KMAX>(lub, ms);
+  emax = __uint_as_float(wave_umax32(__float_as_uint(emax)));
+  const float lc = U - 2.01f * emax;
+  const float L = lc > 0.f ? __uint_as_float(__float_as_uint(lc) - 4u) : 0.f;
+  // Pass B: columns with ub < L are certainly below t (count them); t is the
+  // (ms - below)-th smallest exact value among the columns whose bounds meet
+  // [L, U]; all others lie above U >= t.
+  int below = 0, cnt = 0;
   for (int j0 = 0; j0 < n; j0 += 64) {
     const int j = j0 + lane;
-    bool c = false;
+    bool c = false, lo = false;
     if (j < n) {
       const float g = gi[j];
-      const float lb = j == i ? 0.f : g - gram_err(c2, ni, norms[base + j]);
-      c = lb <= U;
+      const float e = gram_err(c2, ni, norms[base + j]);
+      const float ub = j == i ? 0.f : fmaxf(g + e, 0.f);
+      const float lb = j == i ? 0.f : fmaxf(g - e, 0.f);
+      lo = ub < L;
+      c = !lo && lb <= U;
     }
+    below += __popcll(__ballot(lo));
     const uint64_t m = __ballot(c);
     const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
     if (c && pos < kCoreCands) cand[wave][pos] = (uint16_t)j;
@@ -224,23 +359,31 @@ __global__ __launch_bounds__(256) void core_from_gram_kernel(const float* G, con
   }
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
-#pragma unroll
-  for (int q = 0; q < KMAX; ++q) lst[q] = INFINITY;
+  const int k = max(1, ms - below);
   int bad = 0;
-  if (cnt <= kCoreCands) {
-    for (int k = lane; k < cnt; k += 64) {
-      const int j = cand[wave][k];
-      const float s = j == i ? 0.f : exact_sq(xi, x + (size_t)(base + j) * d, d);
-      list_offer<KMAX>(lst, s);
+  float t;
+  if (cnt <= 64) {
+    // One exact value per lane, each evaluated by the whole wave.
+    float v = INFINITY;
+    for (int q = 0; q < cnt; ++q) {
+      const int j = cand[wave][q];
+      const float s = j == i ? 0.f : wave_exact_sq1(xi, x + (size_t)(base + j) * d, d);
+      if (lane == q) v = s;
     }
+    float one[1] = {v};
+    t = wave_kth<1>(one, k);
   } else {
+    // Many values inside the bounds (ties): every column exactly.
+    float lst[KMAX];
+#pragma unroll
+    for (int q = 0; q < KMAX; ++q) lst[q] = INFINITY;
     for (int j = lane; j < n; j += 64) {
       const float s = j == i ? 0.f : exact_sq(xi, x + (size_t)(base + j) * d, d);
       bad |= !isfinite(s);
       list_offer<KMAX>(lst, s);
     }
+    t = wave_kth<KMAX>(lst, ms);
   }
-  const float t = wave_kth<KMAX>(lst, ms);
   if (lane == 0) core[r] = sqrtf(t);
   if (__any(bad) && lane == 0) atomicOr(status, 1);
 }
@@ -260,49 +403,40 @@ struct GramPrimArgs {
   int32_t* status;
 };
 
-// Exact weight of the pair (a, j): max(core_a, core_j, d_aj) from the
-// points (coordinate order, correctly rounded sqrt); +inf for a < 0 (never
-// relaxed).
-__device__ __forceinline__ float gram_exact_w(const GramPrimArgs& A, const float* xw, const float* cs, int a,
-                                              int j) {
-  if (a < 0) return INFINITY;
-  const float dd = sqrtf(exact_sq(xw + (size_t)a * A.d, xw + (size_t)j * A.d, A.d));
-  return A.unit_core ? dd : fmaxf(fmaxf(cs[a], cs[j]), dd);
+// Exact weights max(core_a, core_j, d_aj) of vertex j and the sources a[0..2]
+// (coordinate-order distances, correctly rounded sqrt), evaluated by the
+// whole wave (uniform arguments, all lanes active); +inf for a < 0.
+__device__ __forceinline__ void wave_exact_w3(const GramPrimArgs& A, const float* xw, const float* cs, const int* a,
+                                              int j, float* w) {
+  const float* rows[3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) rows[p] = xw + (size_t)(a[p] < 0 ? j : a[p]) * A.d;
+  float sq[3];
+  wave_exact_sq<3>(xw + (size_t)j * A.d, rows, A.d, sq);
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const float dd = sqrtf(sq[p]);
+    w[p] = a[p] < 0 ? INFINITY : A.unit_core ? dd : fmaxf(fmaxf(cs[a[p]], cs[j]), dd);
+  }
 }
 
 // Interval state of one unvisited vertex: best weight in [lo, hi], its
 // source sa, and sb >= 0 when a newer relaxation sb tied inside the bounds
-// (the best is then sb's weight if strictly below sa's, else sa's).
+// (the best is then sb's weight if strictly below sa's, else sa's); pc >= 0:
+// a third relaxation by pc left open, to be resolved exactly.
 struct GState {
   float lo, hi;
-  int sa, sb;
+  int sa, sb, pc;
 };
 
-// Exact value and source of a state (a deferred tie resolved).
-__device__ __forceinline__ void gstate_resolve(GState& st, const GramPrimArgs& A, const float* xw, const float* cs,
-                                               int j) {
-  if (st.lo == st.hi && st.sb < 0) return;
-  float w = gram_exact_w(A, xw, cs, st.sa, j);
-  int src = st.sa;
-  if (st.sb >= 0) {
-    const float wb = gram_exact_w(A, xw, cs, st.sb, j);
-    if (wb < w) {
-      w = wb;
-      src = st.sb;
-    }
-  }
-  st.lo = st.hi = w;
-  st.sa = src;
-  st.sb = -1;
-}
+__device__ __forceinline__ bool gstate_exact(const GState& st) { return st.lo == st.hi && st.sb < 0; }
 
 // Relaxation of vertex j by vertex c with the Gram value g (the reference's
 // update: a strictly smaller weight replaces the best; reference
 // hdbscan_gpu/_kernels.py:215-320).  Decided from the bounds when they
 // decide it; a first undecided relaxation is deferred (sb), a second one is
-// resolved exactly at once.
-__device__ __forceinline__ void gstate_relax(GState& st, const GramPrimArgs& A, const float* xw, const float* cs,
-                                             int j, int c, float g, float mc, float E) {
+// left pending (pc) for the wave's exact pass.
+__device__ __forceinline__ void gstate_relax(GState& st, int c, float g, float mc, float E) {
   if (!(mc < st.hi)) return;  // w >= mc >= hi >= best
   const float ls = g - E;
   if (ls >= square_bound(st.hi)) return;  // d >= hi
@@ -322,11 +456,43 @@ __device__ __forceinline__ void gstate_relax(GState& st, const GramPrimArgs& A, 
     st.hi = fminf(st.hi, whi);
     return;
   }
-  gstate_resolve(st, A, xw, cs, j);
-  const float wc = gram_exact_w(A, xw, cs, c, j);
-  if (wc < st.lo) {
-    st.lo = st.hi = wc;
-    st.sa = c;
+  st.pc = c;
+}
+
+// Exact passes of a wave over its slots: every slot whose flag is set is
+// replaced by its exact value and source (sa, then sb and pc in the order
+// they relaxed it, each replacing the best only when strictly smaller).
+// Called with all lanes active; `want` is evaluated per lane.
+template <int VPT, typename Want>
+__device__ __forceinline__ void gstate_exact_pass(GState* S, const GramPrimArgs& A, const float* xw, const float* cs,
+                                                  int T, Want want) {
+  const int lane = threadIdx.x & 63, w0 = threadIdx.x - lane;
+#pragma unroll
+  for (int s = 0; s < VPT; ++s) {
+    uint64_t m = __builtin_amdgcn_ballot_w64(want(s, S[s]));
+    while (m) {
+      const int l = (int)__builtin_ctzll(m);
+      m &= m - 1;
+      const int j = w0 + l + s * T;
+      const int src3[3] = {(int)read_lane((uint32_t)S[s].sa, l), (int)read_lane((uint32_t)S[s].sb, l),
+                           (int)read_lane((uint32_t)S[s].pc, l)};
+      float w3[3];
+      wave_exact_w3(A, xw, cs, src3, j, w3);
+      float w = w3[0];
+      int src = src3[0];
+#pragma unroll
+      for (int p = 1; p < 3; ++p)
+        if (w3[p] < w) {
+          w = w3[p];
+          src = src3[p];
+        }
+      if (lane == l) {
+        S[s].lo = S[s].hi = w;
+        S[s].sa = src;
+        S[s].sb = -1;
+        S[s].pc = -1;
+      }
+    }
   }
 }
 
@@ -416,7 +582,7 @@ __global__ __launch_bounds__(1024) void prim_gram_kernel(GramPrimArgs A) {
   for (int s = 0; s < VPT; ++s) {
     const int j = tid + s * T;
     st[s].lo = st[s].hi = INFINITY;
-    st[s].sa = st[s].sb = -1;
+    st[s].sa = st[s].sb = st[s].pc = -1;
     cr[s] = j < n ? cs[j] : 0.f;
     nr[s] = j < n ? ns[j] : 0.f;
     if (j >= n) vis |= 1u << s;
@@ -443,17 +609,18 @@ __global__ __launch_bounds__(1024) void prim_gram_kernel(GramPrimArgs A) {
     GState s1[VPT], s2[VPT];
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
-      const int j = tid + s * T;
       s1[s] = st[s];
-      if (!((vis >> s) & 1u)) {
-        gstate_relax(s1[s], A, xw, cs, j, cur, g1[s], fmaxf(ccur, cr[s]), gram_err(c2, ncur, nr[s]));
-        s2[s] = s1[s];
-        if (spec && j != pr)
-          gstate_relax(s2[s], A, xw, cs, j, pr, g2[s], fmaxf(cpr, cr[s]), gram_err(c2, npr, nr[s]));
-      } else {
-        s2[s] = s1[s];
-      }
+      if (!((vis >> s) & 1u)) gstate_relax(s1[s], cur, g1[s], fmaxf(ccur, cr[s]), gram_err(c2, ncur, nr[s]));
     }
+    gstate_exact_pass<VPT>(s1, A, xw, cs, T, [](int, const GState& e) { return e.pc >= 0; });
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      s2[s] = s1[s];
+      if (spec && j != pr && !((vis >> s) & 1u))
+        gstate_relax(s2[s], pr, g2[s], fmaxf(cpr, cr[s]), gram_err(c2, npr, nr[s]));
+    }
+    if (spec) gstate_exact_pass<VPT>(s2, A, xw, cs, T, [](int, const GState& e) { return e.pc >= 0; });
     KeyHi m1, m2;
     C sec1, sec2;
     // Reductions of K' (and of K'' when speculating), at most twice each.
@@ -509,16 +676,14 @@ __global__ __launch_bounds__(1024) void prim_gram_kernel(GramPrimArgs A) {
       const bool in1 = !ok1;
       const float lim = in1 ? m1.h : m2.h;
       if (in1) spec = false;
-#pragma unroll
-      for (int s = 0; s < VPT; ++s) {
-        const int j = tid + s * T;
-        if ((vis >> s) & 1u) continue;
-        if (in1) {
-          if (s1[s].lo <= lim) gstate_resolve(s1[s], A, xw, cs, j);
-        } else if (j != pr && s2[s].lo <= lim) {
-          gstate_resolve(s2[s], A, xw, cs, j);
-        }
-      }
+      const int prv = pr;
+      auto open = [&](int s, const GState& e) {
+        return !((vis >> s) & 1u) && e.lo <= lim && !gstate_exact(e) && (in1 || tid + s * T != prv);
+      };
+      if (in1)
+        gstate_exact_pass<VPT>(s1, A, xw, cs, T, open);
+      else
+        gstate_exact_pass<VPT>(s2, A, xw, cs, T, open);
     }
     const bool hit = spec && m1.k.index() == pr;
     const int v1 = m1.k.index();
@@ -533,7 +698,7 @@ __global__ __launch_bounds__(1024) void prim_gram_kernel(GramPrimArgs A) {
         if ((vis >> s) & 1u) bad = 1;
         vis |= 1u << s;
         const size_t o = (size_t)base + step + (j == v1 ? 0 : 1);
-        const bool exact = e.lo == e.hi && e.sb < 0;
+        const bool exact = gstate_exact(e);
         A.edge_u[o] = exact && !(e.lo < rmax) ? 0 : e.sa;
         A.edge_v[o] = j;
         A.edge_w[o] = e.lo;

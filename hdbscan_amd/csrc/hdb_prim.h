@@ -204,6 +204,22 @@ __device__ __forceinline__ uint32_t wave_umin32(uint32_t v) {
   return read_lane(v, 63);
 }
 
+// Wave-wide unsigned maximum of a 32-bit value (the same lane pattern; the
+// unwritten lanes of each move read 0, the identity of the maximum).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_umax_src(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ uint32_t wave_umax32(uint32_t v) {
+  v = max(v, dpp_umax_src<kDppXor1, 0xF>(v));
+  v = max(v, dpp_umax_src<kDppXor2, 0xF>(v));
+  v = max(v, dpp_umax_src<kDppRor4, 0xF>(v));
+  v = max(v, dpp_umax_src<kDppRor8, 0xF>(v));
+  v = max(v, dpp_umax_src<kDppBcast15, 0xA>(v));
+  v = max(v, dpp_umax_src<kDppBcast31, 0xC>(v));
+  return read_lane(v, 63);
+}
+
 // The fp32 candidate key in two 32-bit steps: the smallest weight pattern
 // (six single-instruction levels), then the smallest vertex among the lanes
 // holding it -- the lexicographic minimum of (weight, vertex), i.e. the

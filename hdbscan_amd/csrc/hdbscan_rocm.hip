@@ -215,6 +215,12 @@ void set_gram(Plan& P, int64_t n, int64_t d, int32_t ms, uint32_t lds_limit) {
   P.gram_lds = gram_bytes(dk, rs, n);
 }
 
+// Bytes of one MFMA operand array of the Gram matrix (gram_operand_kernel).
+size_t gram_operand_bytes(int dk, int64_t batch, int64_t n) {
+  const int nq = (3 * dk + 31) / 32;
+  return (size_t)batch * ((n + 15) / 16) * nq * 64 * 16;
+}
+
 int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int flags, Plan* p) {
   if (dtype != HDB_F32 && dtype != HDB_F64) return fail("dtype must be HDB_F32 or HDB_F64");
   if (batch < 1 || d < 1 || n < 2) return fail("X requires nonempty batches/features and at least 2 samples");
@@ -250,8 +256,9 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
     // N=1024, D=16 with core distances; 2.4 % at D=8 without).
     if (vpt == 2 && dk > 0 && (int64_t)dk * rs <= 64) vpt = 1;
     // Points path up to 2048: two slots per thread and 1024 threads beat
-    // four slots and 512 threads (20 % at N=2048, D=16).
-    if (vpt == 4 && n <= 2048 && dk > 0 && (int64_t)2 * dk * rs <= 64 * 4) vpt = 2;
+    // four slots and 512 threads (20 % at N=2048, D=16).  The matrix path
+    // too (speculative Prim 3.12 -> 2.71 ms at N=2048, D=128).
+    if (vpt == 4 && n <= 2048 && (dk == 0 || (int64_t)2 * dk * rs <= 64 * 4)) vpt = 2;
     while (vpt > 1 && (int64_t)vpt * dk * rs > 64 * 4 && (int64_t)(vpt / 2) * 1024 >= n) vpt /= 2;
   }
   while ((int64_t)vpt * 1024 < n) vpt *= 2;
@@ -268,12 +275,11 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
     P.vpt = vpt = 8;
     P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
   }
-  // float32 windows on the matrix path take the MFMA Gram form of the graph
-  // stage unless the exact matrix is asked for (plain, exact graph, a given
-  // matrix, or the exact-matrix Prim variants).
-  P.gram_graph = P.matrix && rs == 4 && d <= 256 && ms <= 64 &&
-                 !(flags & (HDB_FLAG_PLAIN | HDB_FLAG_EXACT_GRAPH | HDB_FLAG_MATRIX_INPUT | HDB_FLAG_ONE_STEP)) &&
-                 want_vpt != 8;
+  // The MFMA Gram form of the graph stage (float32, matrix path) when asked
+  // for: measured slower than the exact matrix at N=2048, D=128 (the
+  // interval-key Prim's rounds cost more than the exact matrix saves).
+  P.gram_graph = P.matrix && rs == 4 && d <= 256 && ms <= 64 && (flags & HDB_FLAG_GRAM_GRAPH) &&
+                 !(flags & (HDB_FLAG_PLAIN | HDB_FLAG_MATRIX_INPUT | HDB_FLAG_ONE_STEP)) && want_vpt != 8;
   if (P.gram_graph) {
     P.gram_dk = d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
     if (P.vpt > 4) return fail("window too large for the vertex layout");
@@ -310,7 +316,9 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.off_matrix = o; o = align256(o + (P.need_matrix ? bn * (size_t)n * rs : 0));
   P.off_cand = o; o = align256(o + (P.core_gram ? bn * kGramCap * 2 : 0));
   P.off_ncand = o; o = align256(o + (P.core_gram ? bn * 4 : 0));
-  const size_t pieces = pieces_needed ? bn * 2 * (size_t)P.gram_dk * 2 : 0;
+  const size_t pieces = P.gram_graph    ? gram_operand_bytes(P.gram_dk, batch, n)
+                        : pieces_needed ? bn * 2 * (size_t)P.gram_dk * 2
+                                        : 0;
   P.off_pa = o; o = align256(o + pieces);
   P.off_pb = o; o = align256(o + pieces);
   P.off_nb = o; o = align256(o + (pieces_needed ? 3 * bn * 4 : 0));
@@ -418,10 +426,18 @@ int launch_gram_prep(hipStream_t s, const Plan& P, const float* x, int64_t point
 template <int DK>
 int launch_gram_matrix(hipStream_t s, const float* x, int64_t batch, int64_t n, int64_t d, const GramBuffers& g,
                        int64_t points, float* G, int32_t* status) {
+  constexpr int NQ = GramShape<DK, 1>::NQ;
+  gram_s16x8* aop = reinterpret_cast<gram_s16x8*>(g.pa);
+  gram_s16x8* bop = reinterpret_cast<gram_s16x8*>(g.pb);
+  const int nblk = (int)((n + 15) / 16);
+  const int64_t entries = batch * nblk * NQ * 64;
+  gram_operand_kernel<DK><<<(unsigned)((entries + 255) / 256), 256, 0, s>>>(x, (int)n, (int)d, entries, nblk, aop, bop);
+  if (int r = check_launch("gram_operand_kernel")) return r;
+  stage_mark(s, "gram_operand_kernel");
   auto fn = gram_matrix_kernel<DK>;
   if (int r = allow_lds((const void*)fn, kGramMatrixLds)) return r;
   const unsigned tps = (unsigned)((n + kGramTile - 1) / kGramTile);
-  fn<<<dim3(tps * tps, (unsigned)batch), 256, kGramMatrixLds, s>>>(x, (int)n, (int)d, g.pa, g.pb, g.nb + 2 * points,
+  fn<<<dim3(tps * tps, (unsigned)batch), 256, kGramMatrixLds, s>>>(x, (int)n, (int)d, aop, bop, g.nb + 2 * points,
                                                                      G, status);
   return check_launch("gram_matrix_kernel");
 }
@@ -440,8 +456,9 @@ int run_gram_graph(hipStream_t s, const Plan& P, const float* x, const Launch& L
   const GramBuffers g = gram_buffers(P, ws);
   float* G = reinterpret_cast<float*>(ws + P.off_matrix);
   float* core = reinterpret_cast<float*>(ws + P.off_core);
-  if (int r = launch_gram_prep(s, P, x, points, d, ws, status)) return r;
-  stage_mark(s, "gram_prep_kernel");
+  gram_norm_kernel<<<(unsigned)((points + 255) / 256), 256, 0, s>>>(x, points, (int)d, g.nb + 2 * points, status);
+  if (int e = check_launch("gram_norm_kernel")) return e;
+  stage_mark(s, "gram_norm_kernel");
   int r = 0;
   switch (P.gram_dk) {
     case 8: r = launch_gram_matrix<8>(s, x, batch, n, d, g, points, G, status); break;
@@ -959,14 +976,14 @@ int hdb_probe_valu(void* stream, int blocks, int iters, float* out, float* ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail("cannot create events");
   valu_probe_kernel<<<(unsigned)blocks, 256, 0, s>>>(iters, 1.0f, out);  // warm-up
-  hipEventRecord(e0, s);
+  (void)hipEventRecord(e0, s);
   valu_probe_kernel<<<(unsigned)blocks, 256, 0, s>>>(iters, 1.0f, out);
-  hipEventRecord(e1, s);
+  (void)hipEventRecord(e1, s);
   int rc = check_launch("valu_probe_kernel");
   if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail("probe synchronisation failed");
   if (!rc && hipEventElapsedTime(ms, e0, e1) != hipSuccess) rc = fail("probe timing failed");
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return rc;
 }
 
@@ -1067,8 +1084,7 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
   // min_samples > 64) and the Gram candidate lists (3 <= min_samples <= 16).
   Plan P;
   const int filter = (flags & HDB_FLAG_PLAIN) ? 0 : HDB_FLAG_FILTER;
-  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_EXACT_GRAPH | filter, &P))
-    return -1;
+  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX | filter, &P)) return -1;
   P.core = true;
   P.core_topk = min_samples <= 64;
   P.kmax = min_samples <= 8 ? 8 : min_samples <= 16 ? 16 : min_samples <= 32 ? 32 : 64;

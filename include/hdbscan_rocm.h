@@ -52,7 +52,7 @@ extern "C" {
 #define HDB_FLAG_PLAIN 64       /* no MFMA Gram filter: every core distance exact */
 #define HDB_FLAG_FILTER 128     /* MFMA Gram filter also on the matrix path */
 #define HDB_FLAG_ONE_STEP 65536 /* matrix-path Prim: one vertex per step, no row speculation */
-#define HDB_FLAG_EXACT_GRAPH 131072 /* matrix path: the exact distance matrix, not the MFMA Gram matrix */
+#define HDB_FLAG_GRAM_GRAPH 262144 /* float32 matrix path: MFMA Gram matrix + interval-key Prim (no exact matrix) */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4; 8 on the speculative
  * matrix path only); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
@@ -139,8 +139,7 @@ int hdb_distances(void *stream, const void *x, int64_t batch, int64_t n,
 
 /* Core distances core[batch][n]: the (min_samples-1)-th smallest distance of
  * each row, self included.  Workspace: hdb_workspace_bytes(batch, n, d,
- * dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_EXACT_GRAPH |
- * HDB_FLAG_FILTER) bytes
+ * dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_FILTER) bytes
  * always suffice (none is used when min_samples <= 64 and the MFMA filter
  * does not apply).  The filter is used unless flags has HDB_FLAG_PLAIN.
  * Replaces reference hdbscan_gpu/_cuda.py:114-118 (cp.partition). */

@@ -7,7 +7,7 @@ import pytest
 import gpu_helpers as G
 from golden_io import records, same_partition, workload_windows
 from oracle import oracle
-from hdbscan_amd._native import FLAG_EXACT_GRAPH, FLAG_ONE_STEP
+from hdbscan_amd._native import FLAG_GRAM_GRAPH, FLAG_ONE_STEP
 
 pytestmark = pytest.mark.gpu
 
@@ -248,7 +248,8 @@ def test_workload_windows(index):
 @pytest.mark.parametrize("flags", [0, 1, 2, 4, 16, 32, 2 | 1, 1 << 8, 2 << 8, 4 << 8, 32 | 4 << 8,
                                    2 | FLAG_ONE_STEP, 2 | 4 << 8, 2 | 8 << 8,
                                    FLAG_ONE_STEP, 1 << 14, 2 << 14, 3 << 14, FLAG_ONE_STEP | 2 << 14,
-                                   1 << 8 | 2 << 14, 4 << 8 | 3 << 14])
+                                   1 << 8 | 2 << 14, 4 << 8 | 3 << 14, 2 | FLAG_GRAM_GRAPH,
+                                   2 | FLAG_GRAM_GRAPH | 4 << 8])
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_path_variants_agree(flags, precision):
     rng = np.random.default_rng(11)
@@ -326,10 +327,11 @@ def test_points_path_edges_bitwise(precision, mcs, ms, eps, vpt, prefilter):
 
 def _edge_cases():
     """Every workload window through the default plan, and the windows of
-    128 dimensions also through the materialised matrix with the MFMA Gram
-    filter forced (HDB_FLAG_MATRIX | HDB_FLAG_FILTER)."""
+    128 dimensions also with the MFMA Gram filter for the core distances
+    (HDB_FLAG_MATRIX | HDB_FLAG_FILTER) and through the MFMA Gram graph
+    stage (HDB_FLAG_GRAM_GRAPH)."""
     cases = [(i, 0) for i in range(len(WORKLOADS))]
-    cases += [(i, 2 | 128) for i, w in enumerate(WORKLOADS) if w[1].shape[1] == 128]
+    cases += [(i, f) for i, w in enumerate(WORKLOADS) if w[1].shape[1] == 128 for f in (2 | 128, FLAG_GRAM_GRAPH)]
     return cases
 
 
@@ -357,15 +359,15 @@ def test_workload_edges_bitwise(index, flags):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("ms", [3, 5, 9, 17, 40, 70])
-@pytest.mark.parametrize("flags", [2 | 64, 2, 2 | 128, 2 | 1 << 8, 2 | 4 << 8, 2 | FLAG_EXACT_GRAPH,
-                                   2 | FLAG_EXACT_GRAPH | 128])
+@pytest.mark.parametrize("flags", [2 | 64, 2, 2 | 128, 2 | FLAG_GRAM_GRAPH, 2 | FLAG_GRAM_GRAPH | 1 << 8,
+                                   2 | FLAG_GRAM_GRAPH | 2 << 8, 2 | FLAG_GRAM_GRAPH | 4 << 8])
 def test_matrix_path_core_selection_edges(precision, ms, flags):
-    """float32 matrix path: the MFMA Gram matrix with interval-key Prim and
-    core distances from its bounds (flags 2, any vertices per thread; the
-    stress windows put separations far inside the bound, so most choices
-    are resolved exactly), or the exact matrix (HDB_FLAG_EXACT_GRAPH, plain)
-    with core distances selected from it or through the Gram filter; float64
-    always takes the exact matrix.  Prim edge weights (which contain the core
+    """The exact matrix (flags 2, plain) with core distances selected from it
+    or through the Gram filter (128), or, for float32, the MFMA Gram matrix
+    with interval-key Prim and core distances from its bounds
+    (HDB_FLAG_GRAM_GRAPH, any vertices per thread; the stress windows put
+    separations far inside the bound, so many choices are resolved exactly;
+    float64 ignores the flag).  Prim edge weights (which contain the core
     distances) bitwise."""
     for name, x in prefilter_windows(precision).items():
         labels, status, ex = G.cluster_export(x, 5, ms, 0.0, precision, flags=flags)
