@@ -77,36 +77,52 @@ HDB_HD TreeLayout tree_layout(int n, int real_size) {
   return L;
 }
 
-// The windows of one launch.  Dense: window w holds points w*n .. w*n+n-1.
-// Ragged: window w holds points off[w] .. off[w+1]-1 and a launch covers the
-// windows whose size lies in (lo, n]; a workgroup of another window exits
-// before any barrier.
+// The windows of one launch.  Dense: item w is window w, points w*n ..
+// w*n+n-1.  Ragged: item i is window list[i] (the launch's windows, largest
+// first), points off[w] .. off[w+1]-1.  Persistent launches (queue != null)
+// let each workgroup claim items from the device counter *queue until all
+// `count` items are taken.
 struct Windows {
-  const int64_t* off;  // [count + 1] point offsets, or null (dense)
-  int n;               // dense size, or the largest size of the launch
-  int lo;              // ragged: sizes <= lo belong to another launch
-  int count;           // number of windows (ragged)
-  __device__ __forceinline__ bool get(int w, int64_t& base, int& nw) const {
+  const int64_t* off;   // [windows + 1] point offsets, or null (dense)
+  const int32_t* list;  // ragged: the launch's windows in claim order
+  const int64_t* moff;  // ragged matrix path: element offset of each window's n x n matrix
+  int n;                // dense size, or the largest size of the launch
+  int count;            // items of the launch
+  int* queue;           // claim counter of a persistent launch, or null
+  __device__ __forceinline__ void get(int i, int& w, int64_t& base, int& nw) const {
     if (!off) {
-      base = (int64_t)w * n;
+      w = i;
+      base = (int64_t)i * n;
       nw = n;
-      return true;
+      return;
     }
+    w = list[i];
     base = off[w];
     nw = (int)(off[w + 1] - base);
-    return nw > lo && nw <= n;
   }
-  // Window of point p (ragged: binary search over the offsets).
-  __device__ __forceinline__ int window_of(int64_t p) const {
-    if (!off) return (int)(p / n);
-    int a = 0, b = count;
-    while (b - a > 1) {
-      const int m = (a + b) >> 1;
-      if (off[m] <= p) a = m; else b = m;
-    }
-    return a;
-  }
+  // Element offset of window w's distance matrix (dense: base * n).
+  __device__ __forceinline__ int64_t matrix(int w, int64_t base) const { return moff ? moff[w] : base * n; }
 };
+
+// Runs body(item) for every item of a launch: one item per workgroup
+// (blockIdx.x), or, with a queue, items claimed one at a time until none is
+// left (every thread of the workgroup takes part in each body call).
+template <typename Body>
+__device__ __forceinline__ void for_each_item(const Windows& win, Body body) {
+  if (!win.queue) {
+    body((int)blockIdx.x);
+    return;
+  }
+  __shared__ int sh_item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) sh_item = atomicAdd(win.queue, 1);
+    __syncthreads();
+    const int item = sh_item;
+    if (item >= win.count) break;
+    body(item);
+  }
+}
 
 // Global tree arena and cluster slots of ragged launches are addressed per
 // point: window w with first point base uses bytes

@@ -219,7 +219,9 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
   const int row0 = blockIdx.x * W * WR;
   int64_t wbase;
   int n;
-  if (!win.get(blockIdx.y, wbase, n) || row0 >= n) return;
+  int wid;
+  win.get(blockIdx.y, wid, wbase, n);
+  if (row0 >= n) return;
   uint16_t* bcol = reinterpret_cast<uint16_t*>(lds);          // [TJ][KS] bf16
   // [2][n + kGramPad]: upper then lower norm terms, +inf past the window.
   float* nbound = reinterpret_cast<float*>(bcol + TJ * KS);
@@ -574,12 +576,14 @@ __global__ __launch_bounds__(256) void core_exact_kernel(const float* x, const f
                                                          float* core, int32_t* status) {
   int64_t base;
   int n;
-  if (!win.get(blockIdx.y, base, n) || (int)(blockIdx.x * 256) >= n) return;
+  int wid;
+  win.get(blockIdx.y, wid, base, n);
+  if ((int)(blockIdx.x * 256) >= n) return;
   const int i = blockIdx.x * 256 + threadIdx.x;
   int bad = 0;
   if (i < n) {
     const int64_t row = base + i;
-    const float* di = dist ? dist + (size_t)row * n : nullptr;
+    const float* di = dist ? dist + win.matrix(wid, base) + (size_t)i * n : nullptr;
     core[row] = exact_row<KMAX>(x + (size_t)base * d, di, n, d, ms, i, cand + row * kGramCap, ncand[row], bad);
   }
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
@@ -598,7 +602,8 @@ __global__ __launch_bounds__(1024) void core_exact_lds_kernel(const float* x, Wi
   extern __shared__ __align__(16) float xs[];
   int64_t base;
   int n;
-  if (!win.get(blockIdx.x, base, n)) return;
+  int wid;
+  win.get(blockIdx.x, wid, base, n);
   const float* xb = x + (size_t)base * d;
   const int total = n * d;
   if (((((uintptr_t)xb) & 15) == 0) && (total & 3) == 0) {

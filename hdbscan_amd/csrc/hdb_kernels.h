@@ -22,28 +22,34 @@ __global__ __launch_bounds__(256) void check_finite_kernel(const Real* x, size_t
 // ascending, then takes the correctly rounded square root.  x_i - x_j is the
 // exact negation of x_j - x_i, so the mirror holds the same bits the
 // reference's triangle-and-mirror kernel writes (reference
-// hdbscan_gpu/_kernels.py:45-73).  Grid: (nt (nt + 1) / 2, batch) for nt
-// tiles per side; block t is the t-th tile of the upper triangle in row order.
+// hdbscan_gpu/_kernels.py:45-73).  Grid: (nt (nt + 1) / 2, items) for nt
+// tiles per side of the launch's largest window; block t is the t-th tile of
+// the upper triangle in row order (blocks past a smaller window's triangle
+// exit).  Window w's matrix starts at element win.matrix(w, base).
 // Input check: the diagonal tiles stage every coordinate of the window
 // exactly once, so they OR HDB_STATUS_NONFINITE (2) into *status for a
 // non-finite coordinate (the reference's input check, hdbscan_gpu/_cuda.py:
 // 96-99, which outranks the overflow bit 1 set for a non-finite distance).
 template <typename Real>
-__global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int d, Real* out, int32_t* status) {
+__global__ __launch_bounds__(256) void distance_kernel(const Real* x, Windows win, int d, Real* out,
+                                                       int32_t* status) {
   constexpr int TILE = 64, KC = 16;
   __shared__ Real si[KC][TILE];
   __shared__ Real sj[KC][TILE];
   __shared__ Real st[TILE][TILE + 1];
+  int64_t base;
+  int n, w;
+  win.get(blockIdx.y, w, base, n);
   const int nt = (n + TILE - 1) / TILE;
+  if ((int)blockIdx.x >= nt * (nt + 1) / 2) return;
   int t = blockIdx.x, rt = 0;
   while (t >= nt - rt) {
     t -= nt - rt;
     ++rt;
   }
-  const int b = blockIdx.y;
   const int ti = rt * TILE, tj = (rt + t) * TILE;
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  const Real* xb = x + (size_t)b * n * d;
+  const Real* xb = x + (size_t)base * d;
   const bool diag = ti == tj;
   int nonfinite = 0;
   Real tot[4][4];
@@ -79,7 +85,7 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
     }
   }
   int bad = 0;
-  Real* ob = out + (size_t)b * n * n;
+  Real* ob = out + win.matrix(w, base);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int gi = ti + ty * 4 + r;
@@ -106,6 +112,44 @@ __global__ __launch_bounds__(256) void distance_kernel(const Real* x, int n, int
   }
   if (__any(bad) && (tid & 63) == 0) atomicOr(status, 1);
   if (__any(nonfinite) && (tid & 63) == 0) atomicOr(status, 2);
+}
+
+// Launch lists of a ragged batch.  Window w of size n_w belongs to bucket k
+// (sizes up to top[k], above top[k-1]) and goes to lists[start[k] + rank],
+// rank = its position among the bucket's windows by size, largest first
+// (equal sizes by index): the claim order of the bucket's launches, longest
+// first.  Windows of buckets flagged in matrix_mask get the element offset of
+// their n x n distance matrix (those windows in index order).  The launches'
+// claim counters are zeroed here (no memset node in a captured graph).
+struct RaggedLists {
+  int32_t start[8];
+  int32_t top[8];
+  uint32_t matrix_mask;
+  int32_t queues;
+};
+
+__global__ __launch_bounds__(256) void ragged_lists_kernel(const int64_t* off, int windows, RaggedLists rl,
+                                                           int32_t* lists, int64_t* moff, int32_t* queue) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g < rl.queues) queue[g] = 0;
+  if (g >= windows) return;
+  auto bucket = [&](int64_t nv) {
+    int k = 0;
+    while (k < 7 && nv > rl.top[k]) ++k;
+    return k;
+  };
+  const int64_t n = off[g + 1] - off[g];
+  const int k = bucket(n);
+  int rank = 0;
+  int64_t mo = 0;
+  for (int v = 0; v < windows; ++v) {
+    const int64_t nv = off[v + 1] - off[v];
+    const int kv = bucket(nv);
+    rank += kv == k && (nv > n || (nv == n && v < g));
+    if (v < g && ((rl.matrix_mask >> kv) & 1u)) mo += nv * nv;
+  }
+  lists[rl.start[k] + rank] = g;
+  moff[g] = mo;
 }
 
 // Issue-rate probe for the exact distance's instruction mix: per coordinate
@@ -183,7 +227,9 @@ __global__ __launch_bounds__(256) void core_topk_kernel(const Real* x, Windows w
   __shared__ Real tile[TJ * KC];
   int64_t base;
   int n;
-  if (!win.get(blockIdx.y, base, n) || (int)(blockIdx.x * blockDim.x) >= n) return;
+  int wid;
+  win.get(blockIdx.y, wid, base, n);
+  if ((int)(blockIdx.x * blockDim.x) >= n) return;
   const int tid = threadIdx.x;
   const int i = blockIdx.x * blockDim.x + tid;
   const Real* xb = x + (size_t)base * d;
@@ -275,12 +321,16 @@ template <> struct BitsOf<double> {
 // row; the row is staged in LDS and the (ms-1)-th smallest value is built
 // bit by bit from the top (non-negative IEEE values order like their bits).
 template <typename Real>
-__global__ __launch_bounds__(64) void core_select_kernel(const Real* dist, int n, int ms, Real* core) {
+__global__ __launch_bounds__(64) void core_select_kernel(const Real* dist, Windows win, int ms, Real* core) {
   using U = typename BitsOf<Real>::U;
   extern __shared__ __align__(16) unsigned char lds[];
   Real* row = reinterpret_cast<Real*>(lds);
-  const size_t r = blockIdx.x;
-  const Real* src = dist + r * n;
+  int64_t base;
+  int n, w;
+  win.get(blockIdx.y, w, base, n);
+  if ((int)blockIdx.x >= n) return;
+  const size_t r = (size_t)base + blockIdx.x;
+  const Real* src = dist + win.matrix(w, base) + (size_t)blockIdx.x * n;
   const int lane = threadIdx.x;
   for (int j = lane; j < n; j += 64) row[j] = src[j];
   __syncthreads();
@@ -302,14 +352,17 @@ __global__ __launch_bounds__(64) void core_select_kernel(const Real* dist, int n
 // (coalesced reads straight from memory); the ms-th smallest of the union
 // of the 64 sorted lists is then found by popping wave minima (equal heads
 // pop together, so the value at which the count first reaches ms is the
-// order statistic).  Grid: batch * n waves.
+// order statistic).  Grid: (ceil(n / 4), items), one wave per row.
 template <typename Real, int KMAX>
-__global__ __launch_bounds__(256) void core_select_list_kernel(const Real* dist, int64_t rows, int n, int ms,
-                                                               Real* core) {
+__global__ __launch_bounds__(256) void core_select_list_kernel(const Real* dist, Windows win, int ms, Real* core) {
   const int lane = threadIdx.x & 63;
-  const int64_t r = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-  if (r >= rows) return;  // uniform per wave
-  const Real* src = dist + (size_t)r * n;
+  int64_t base;
+  int n, w;
+  win.get(blockIdx.y, w, base, n);
+  const int i = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (i >= n) return;  // uniform per wave
+  const int64_t r = base + i;
+  const Real* src = dist + win.matrix(w, base) + (size_t)i * n;
   Real lst[KMAX];
 #pragma unroll
   for (int q = 0; q < KMAX; ++q) lst[q] = Real(INFINITY);
@@ -391,18 +444,16 @@ struct TreeArgs {
   void* export_reals;    // [B][4][n] or null
 };
 
-// Tree stages of one window per workgroup.  Lane 0 runs the sequential
-// stages; the workgroup stages inputs and writes labels and exports.
+// Tree stages of one window (item of the launch) by one workgroup.
 template <typename Real, bool LDSA>
-__global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
+__device__ __forceinline__ void tree_window(const TreeArgs& a, int item) {
   extern __shared__ __align__(16) unsigned char lds[];
-  __shared__ int sh_next;
   __shared__ int sh_counters[4];
   __shared__ long long sh_clk[6];
-  const int b = blockIdx.x, tid = threadIdx.x, T = blockDim.x;
+  const int tid = threadIdx.x, T = blockDim.x;
   int64_t p0;
-  int n;
-  if (!a.win.get(b, p0, n)) return;
+  int n, b;
+  a.win.get(item, b, p0, n);
   int32_t* lab = a.labels + (size_t)p0;
   if (a.wflag[b]) {
     for (int i = tid; i < n; i += T) lab[i] = -1;
@@ -517,6 +568,14 @@ __global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
     const int pp = t.pparent[i];
     lab[i] = pp >= n ? resolved[pp - n] : -1;
   }
+}
+
+// Tree stages, one window per workgroup or, with a queue, windows claimed
+// one after another.  Lane 0 runs the sequential stages; the workgroup
+// stages inputs and writes labels and exports.
+template <typename Real, bool LDSA>
+__global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
+  for_each_item(a.win, [&](int item) { tree_window<Real, LDSA>(a, item); });
 }
 
 }  // namespace hdb

@@ -392,13 +392,12 @@ struct PrimArgs {
 // certified FMA Gram prefilter (a separate instantiation keeps the plain
 // path free of its branches).
 template <typename Real, int VPT, int DK, int MODE, bool XLDS>
-__global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
+__device__ __forceinline__ void prim_window(const PrimArgs& a, int item) {
   using C = Cand<Real>;
   constexpr bool PTS = MODE != 1;  // points path (MODE 0, or 2 with the Gram prefilter)
-  const int b = blockIdx.x;
   int64_t base;
-  int n;
-  if (!a.win.get(b, base, n)) return;
+  int n, b;
+  a.win.get(item, b, base, n);
   const int tid = threadIdx.x, T = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
   const int d = a.d;
@@ -415,7 +414,7 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   Real* xs = reinterpret_cast<Real*>(lds);
   Real* cs = reinterpret_cast<Real*>(lds + (XLDS ? (size_t)n * DK * sizeof(Real) : 0));
   const Real* xg = static_cast<const Real*>(a.x) + (size_t)base * (PTS ? d : 0);
-  const Real* dg = static_cast<const Real*>(a.dist) + (size_t)base * (MODE == 1 ? n : 0);
+  const Real* dg = static_cast<const Real*>(a.dist) + (MODE == 1 ? a.win.matrix(b, base) : 0);
   const Real* cg = static_cast<const Real*>(a.core) + (size_t)base;
   if (tid == 0) {
     sh_err = 0;
@@ -701,6 +700,12 @@ __global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
   }
 }
 
+// One window per workgroup, or windows claimed from the launch's queue.
+template <typename Real, int VPT, int DK, int MODE, bool XLDS>
+__global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
+  for_each_item(a.win, [&](int item) { prim_window<Real, VPT, DK, MODE, XLDS>(a, item); });
+}
+
 template <typename Real>
 __device__ __forceinline__ bool cand_eq(const Cand<Real>& x, const Cand<Real>& y) {
   return !x.less(y) && !y.less(x);
@@ -734,12 +739,11 @@ __device__ __forceinline__ void cand_top2(Cand<Real>& first, Cand<Real>& second,
 // final keys predicts the next round (after a miss the next round reads one
 // row and yields the prediction).
 template <typename Real, int VPT>
-__global__ __launch_bounds__(VPT == 8 ? 512 : 1024) void prim_spec_kernel(PrimArgs a) {
+__device__ __forceinline__ void prim_spec_window(const PrimArgs& a, int item) {
   using C = Cand<Real>;
-  const int b = blockIdx.x;
   int64_t base;
-  int n;
-  if (!a.win.get(b, base, n)) return;
+  int n, b;
+  a.win.get(item, b, base, n);
   const int tid = threadIdx.x, T = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
   const bool unit = a.unit_core != 0;
@@ -747,7 +751,7 @@ __global__ __launch_bounds__(VPT == 8 ? 512 : 1024) void prim_spec_kernel(PrimAr
   __shared__ C red[2][3][16];  // [round parity][min K', min K'' (or K'), second][wave]
   __shared__ int sh_err;
   Real* cs = reinterpret_cast<Real*>(lds);
-  const Real* dg = static_cast<const Real*>(a.dist) + (size_t)base * n;
+  const Real* dg = static_cast<const Real*>(a.dist) + a.win.matrix(b, base);
   const Real* cg = static_cast<const Real*>(a.core) + (size_t)base;
   if (tid == 0) sh_err = 0;
   if (!unit)
@@ -894,6 +898,11 @@ __global__ __launch_bounds__(VPT == 8 ? 512 : 1024) void prim_spec_kernel(PrimAr
     a.wflag[b] = sh_err;
     if (sh_err) atomicOr(a.status, sh_err);
   }
+}
+
+template <typename Real, int VPT>
+__global__ __launch_bounds__(VPT == 8 ? 512 : 1024) void prim_spec_kernel(PrimArgs a) {
+  for_each_item(a.win, [&](int item) { prim_spec_window<Real, VPT>(a, item); });
 }
 
 }  // namespace hdb

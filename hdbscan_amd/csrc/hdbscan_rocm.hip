@@ -17,6 +17,7 @@
 //   norm terms (filter) | Prim repacking records (points path)
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -135,16 +136,40 @@ struct Plan {
 // window count, the points covered and the largest window.
 struct Launch {
   Windows win;
-  int64_t count = 0;   // workgroups along the window axis
+  int64_t count = 0;   // items (windows) of this launch
   int64_t points = 0;  // points of the whole batch (per-point buffers)
   int64_t nmax = 0;    // largest window of this launch
+  int* prim_queue = nullptr;  // claim counters of persistent Prim / tree launches (ragged)
+  int* tree_queue = nullptr;
+  int64_t persist = 0;        // workgroups of a persistent launch
 };
+
+// Compute units of the current device (256 on MI355X when none is visible).
+int device_cus() {
+  static std::mutex mu;
+  static int cached_device = -1, cached = 256;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev != cached_device) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cached = v;
+    else (void)hipGetLastError();
+    cached_device = dev;
+  }
+  return cached;
+}
 
 Launch dense_launch(int64_t batch, int64_t n) {
   Launch L;
   L.win.off = nullptr;
+  L.win.list = nullptr;
+  L.win.moff = nullptr;
+  L.win.queue = nullptr;
   L.win.n = (int)n;
-  L.win.lo = 0;
   L.win.count = (int)batch;
   L.count = batch;
   L.points = batch * n;
@@ -336,10 +361,10 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
 }
 
 template <typename Real>
-int launch_distances(hipStream_t s, const Real* x, int64_t batch, int64_t n, int64_t d, Real* out, int32_t* status) {
-  const unsigned nt = (unsigned)((n + 63) / 64);
-  const dim3 grid(nt * (nt + 1) / 2, (unsigned)batch);
-  distance_kernel<Real><<<grid, 256, 0, s>>>(x, (int)n, (int)d, out, status);
+int launch_distances(hipStream_t s, const Real* x, const Launch& L, int64_t d, Real* out, int32_t* status) {
+  const unsigned nt = (unsigned)((L.nmax + 63) / 64);
+  const dim3 grid(nt * (nt + 1) / 2, (unsigned)L.count);
+  distance_kernel<Real><<<grid, 256, 0, s>>>(x, L.win, (int)d, out, status);
   return check_launch("distance_kernel");
 }
 
@@ -560,25 +585,22 @@ int launch_core(hipStream_t s, const Plan& P, const Real* x, const Launch& L, in
       default: return launch_topk_k<Real, 64>(s, x, L, d, ms, core, P.core_dk, status);
     }
   }
-  // Materialised matrices exist for dense batches only.
-  if (L.win.off) return fail("internal: ragged batch on the matrix path");
-  const int64_t batch = L.count, n = L.nmax;
+  const int64_t n = L.nmax;
   if (!matrix_ready) {
-    if (int r = launch_distances<Real>(s, x, batch, n, d, matrix, status)) return r;
+    if (int r = launch_distances<Real>(s, x, L, d, matrix, status)) return r;
   }
-  const int64_t rows = batch * n;
-  const unsigned blocks = (unsigned)((rows + 3) / 4);
+  const dim3 grid((unsigned)((n + 3) / 4), (unsigned)L.count);
   if (ms <= 8) {
-    core_select_list_kernel<Real, 8><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+    core_select_list_kernel<Real, 8><<<grid, 256, 0, s>>>(matrix, L.win, ms, core);
   } else if (ms <= 16) {
-    core_select_list_kernel<Real, 16><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+    core_select_list_kernel<Real, 16><<<grid, 256, 0, s>>>(matrix, L.win, ms, core);
   } else if (ms <= 32) {
-    core_select_list_kernel<Real, 32><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+    core_select_list_kernel<Real, 32><<<grid, 256, 0, s>>>(matrix, L.win, ms, core);
   } else if (ms <= 64) {
-    core_select_list_kernel<Real, 64><<<blocks, 256, 0, s>>>(matrix, rows, (int)n, ms, core);
+    core_select_list_kernel<Real, 64><<<grid, 256, 0, s>>>(matrix, L.win, ms, core);
   } else {
     const uint32_t bytes = (uint32_t)(n * sizeof(Real));
-    core_select_kernel<Real><<<(unsigned)rows, 64, bytes, s>>>(matrix, (int)n, ms, core);
+    core_select_kernel<Real><<<dim3((unsigned)n, (unsigned)L.count), 64, bytes, s>>>(matrix, L.win, ms, core);
   }
   return check_launch("core_select_kernel");
 }
@@ -675,8 +697,7 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   if (!given_dist && !prim_done) {
     bool matrix_ready = false;
     if (P.matrix) {
-      if (L.win.off) return fail("internal: ragged batch on the matrix path");
-      if (int r = launch_distances<Real>(s, x, L.count, L.nmax, d, matrix, status)) return r;
+      if (int r = launch_distances<Real>(s, x, L, d, matrix, status)) return r;
       matrix_ready = true;
       dist = matrix;
       stage_mark(s, "distance_kernel");
@@ -704,7 +725,9 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   pa.wflag = wflag;
   pa.status = status;
   if (!prim_done) {
-    if (int r = launch_prim<Real>(s, P, L.count, pa)) return r;
+    pa.win.queue = L.prim_queue;
+    const int64_t grid = L.prim_queue ? std::min(L.count, L.persist) : L.count;
+    if (int r = launch_prim<Real>(s, P, grid, pa)) return r;
     stage_mark(s, "prim_kernel");
   }
   TreeArgs ta;
@@ -725,7 +748,8 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   ta.status = status;
   ta.export_ints = exp_i;
   ta.export_reals = exp_r;
-  if (int r = launch_tree<Real>(s, P, L.count, ta)) return r;
+  ta.win.queue = L.tree_queue;
+  if (int r = launch_tree<Real>(s, P, L.tree_queue ? std::min(L.count, L.persist) : L.count, ta)) return r;
   stage_mark(s, "tree_kernel");
   return 0;
 }
@@ -748,13 +772,16 @@ constexpr int kSideStreams = 4;
 
 struct RaggedPlan {
   int64_t windows = 0, points = 0;
-  int64_t count[kBuckets] = {}, nmax[kBuckets] = {};
+  int64_t count[kBuckets] = {}, nmax[kBuckets] = {}, start[kBuckets + 1] = {};
   Plan plan[kBuckets];
   bool used[kBuckets] = {};
   bool gram = false, global_arena = false;
-  int gram_dk = 0;
+  int gram_bucket = -1;
+  uint32_t matrix_mask = 0;
+  int64_t matrix_elems = 0;
   size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0;
-  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, off_pack = 0, total = 0;
+  size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, off_pack = 0;
+  size_t off_list = 0, off_moff = 0, off_queue = 0, off_matrix = 0, total = 0;
 };
 
 int bucket_of(int64_t n) {
@@ -763,10 +790,18 @@ int bucket_of(int64_t n) {
   return k;
 }
 
+// Per-bucket tuning flags (vertices per thread, Gram row sets, repacking,
+// arena): bits of these masks from a tuning table replace the call's.
+constexpr int kTuneMask = 0xFF00 | HDB_FLAG_GLOBAL_ARENA | HDB_FLAG_NO_POINT_LDS;
+std::mutex g_tune_mutex;
+int g_tune[kBuckets] = {};
+bool g_tune_set = false;
+
 int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, int32_t ms, int flags,
                      RaggedPlan* out) {
   if (!off) return fail("null pointer argument");
   if (windows < 1) return fail("X requires nonempty batches/features and at least 2 samples");
+  if (windows > (1 << 20)) return fail("at most 2^20 windows per ragged call");
   if (off[0] != 0) return fail("window offsets must start at 0");
   RaggedPlan R;
   R.windows = windows;
@@ -780,25 +815,41 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
     R.count[k] += 1;
     R.nmax[k] = std::max<int64_t>(R.nmax[k], n);
   }
+  for (int k = 0; k < kBuckets; ++k) R.start[k + 1] = R.start[k] + R.count[k];
   R.points = off[windows];
   const int rs = dtype == HDB_F32 ? 4 : 8;
+  int tune[kBuckets] = {};
+  bool tuned = false;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    tuned = g_tune_set;
+    for (int k = 0; k < kBuckets; ++k) tune[k] = g_tune[k];
+  }
+  // The Gram graph stage lays matrices out densely; ragged buckets on the
+  // matrix path use the exact matrix.
+  const int base_flags = flags & ~HDB_FLAG_GRAM_GRAPH;
   for (int k = 0; k < kBuckets; ++k) {
     if (!R.used[k]) continue;
-    if (make_plan(R.count[k], R.nmax[k], d, dtype, ms, flags, &R.plan[k])) return -1;
+    const int fk = tuned ? (base_flags & ~kTuneMask) | (tune[k] & kTuneMask) : base_flags;
+    if (make_plan(R.count[k], R.nmax[k], d, dtype, ms, fk, &R.plan[k])) return -1;
     const Plan& P = R.plan[k];
-    if (P.matrix)
-      return fail("ragged batches need the points path: at most 16 dimensions (float32) or 8 (float64) "
-                  "for windows above 1024 points, 64 / 32 below");
+    if (P.matrix) R.matrix_mask |= 1u << k;
     if (P.core_gram) {
       R.gram = true;
-      R.gram_dk = P.gram_dk;
+      R.gram_bucket = k;
     }
     if (!P.tree_lds) R.global_arena = true;
   }
   for (int k = 0; k < kBuckets; ++k)
-    if (R.used[k] && R.gram && !R.plan[k].core_gram) return fail("internal: mixed Gram plans in a ragged batch");
+    if (R.used[k] && R.plan[k].core_gram && R.plan[k].gram_dk != R.plan[R.gram_bucket].gram_dk)
+      return fail("internal: Gram paddings differ between buckets");
+  for (int64_t w = 0; w < windows; ++w) {
+    const int64_t n = off[w + 1] - off[w];
+    if ((R.matrix_mask >> bucket_of(n)) & 1u) R.matrix_elems += n * n;
+  }
   const size_t pts = (size_t)R.points;
   const bool core = ms > 2;
+  const int gram_dk = R.gram ? R.plan[R.gram_bucket].gram_dk : 0;
   size_t o = 0;
   R.off_eu = o; o = align256(o + pts * 4);
   R.off_ev = o; o = align256(o + pts * 4);
@@ -810,11 +861,15 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
   o = align256(o + pts * (rs == 4 ? sizeof(ClusterSlot<float>) : sizeof(ClusterSlot<double>)));
   R.off_cand = o; o = align256(o + (R.gram ? pts * kGramCap * 2 : 0));
   R.off_ncand = o; o = align256(o + (R.gram ? pts * 4 : 0));
-  const size_t pieces = R.gram ? pts * 2 * (size_t)R.gram_dk * 2 : 0;
+  const size_t pieces = R.gram ? pts * 2 * (size_t)gram_dk * 2 : 0;
   R.off_pa = o; o = align256(o + pieces);
   R.off_pb = o; o = align256(o + pieces);
   R.off_nb = o; o = align256(o + (R.gram ? 3 * pts * 4 : 0));
   R.off_pack = o; o = align256(o + pts * (rs == 4 ? sizeof(PackRec<float>) : sizeof(PackRec<double>)));
+  R.off_list = o; o = align256(o + (size_t)windows * 4);
+  R.off_moff = o; o = align256(o + (size_t)windows * 8);
+  R.off_queue = o; o = align256(o + 2 * kBuckets * 4);
+  R.off_matrix = o; o = align256(o + (size_t)R.matrix_elems * rs);
   R.total = o;
   // Point the bucket plans at the shared buffers (per-point addressing).
   for (int k = 0; k < kBuckets; ++k) {
@@ -822,7 +877,7 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
     P.off_eu = R.off_eu; P.off_ev = R.off_ev; P.off_ew = R.off_ew; P.off_flag = R.off_flag;
     P.off_core = R.off_core; P.off_arena = R.off_arena; P.off_slots = R.off_slots;
     P.off_cand = R.off_cand; P.off_ncand = R.off_ncand; P.off_pa = R.off_pa; P.off_pb = R.off_pb;
-    P.off_nb = R.off_nb; P.off_matrix = 0; P.total = R.total; P.off_pack = R.off_pack;
+    P.off_nb = R.off_nb; P.off_matrix = R.off_matrix; P.total = R.total; P.off_pack = R.off_pack;
     P.arena_stride = 0;
     P.slot_stride = 0;
   }
@@ -862,6 +917,11 @@ int side_streams(hipStream_t caller, SideStreams** out) {
   return 0;
 }
 
+// One ragged call: the launch lists (device, largest windows first), the
+// Gram operands of the whole batch, then every bucket's pipeline on the
+// internal streams.  Prim and tree launches are persistent: a few
+// workgroups per compute unit claim the bucket's windows from a device
+// counter in list order.
 template <typename Real>
 int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t* off_dev, int64_t d,
                long long mcs, int ms, double eps, int32_t* labels, unsigned char* ws, int32_t* status) {
@@ -871,14 +931,26 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
   const bool timers = g_timers.on;
   g_timers.on = false;  // stage events assume one stream
   int err = 0;
-  if constexpr (sizeof(Real) == 4) {
-    if (R.gram) {
-      int k0 = 0;
-      while (!R.used[k0]) ++k0;
-      err = launch_gram_prep(s, R.plan[k0], x, R.points, d, ws, status);
+  int32_t* lists = reinterpret_cast<int32_t*>(ws + R.off_list);
+  int64_t* moff = reinterpret_cast<int64_t*>(ws + R.off_moff);
+  int32_t* queues = reinterpret_cast<int32_t*>(ws + R.off_queue);
+  {
+    RaggedLists rl;
+    for (int k = 0; k < 8; ++k) {
+      rl.start[k] = k <= kBuckets ? (int32_t)R.start[std::min(k, kBuckets)] : 0;
+      rl.top[k] = k < kBuckets ? kBucketTop[k] : HDB_MAX_SAMPLES;
     }
+    rl.matrix_mask = R.matrix_mask;
+    rl.queues = 2 * kBuckets;
+    const unsigned blocks = (unsigned)((std::max<int64_t>(R.windows, 2 * kBuckets) + 255) / 256);
+    ragged_lists_kernel<<<blocks, 256, 0, s>>>(off_dev, (int)R.windows, rl, lists, moff, queues);
+    err = check_launch("ragged_lists_kernel");
+  }
+  if constexpr (sizeof(Real) == 4) {
+    if (!err && R.gram) err = launch_gram_prep(s, R.plan[R.gram_bucket], x, R.points, d, ws, status);
   }
   if (!err && hipEventRecord(S->fork, s) != hipSuccess) err = fail("event record failed");
+  const int cus = device_cus();
   // The bucket of the largest windows is the critical path (its Prim runs
   // longest): it gets the first internal stream to itself, and the other
   // buckets, spread over the remaining streams, start once its core
@@ -898,12 +970,17 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
     }
     Launch L;
     L.win.off = off_dev;
+    L.win.list = lists + R.start[k];
+    L.win.moff = moff;
+    L.win.queue = nullptr;
     L.win.n = (int)R.nmax[k];
-    L.win.lo = k == 0 ? 0 : kBucketTop[k - 1];
-    L.win.count = (int)R.windows;
-    L.count = R.windows;
+    L.win.count = (int)R.count[k];
+    L.count = R.count[k];
     L.points = R.points;
     L.nmax = R.nmax[k];
+    L.prim_queue = queues + 2 * k;
+    L.tree_queue = queues + 2 * k + 1;
+    L.persist = (int64_t)cus * std::max(1, 1024 / std::max(64, R.plan[k].threads));
     err = run_cluster<Real>(t, R.plan[k], x, nullptr, nullptr, L, d, mcs, ms, ms <= 2, eps, -100000, labels, ws,
                             status, nullptr, nullptr, false, used == 0 ? S->head : nullptr);
     ++used;
@@ -915,6 +992,81 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
   }
   g_timers.on = timers;
   return err;
+}
+
+// Ragged calls are captured into a HIP graph on an internal stream the first
+// time a call signature (buffers, sizes, parameters, window offsets) is seen
+// and the graph is replayed onto the caller's stream afterwards: the host
+// then enqueues one graph launch instead of some thirty kernel launches and
+// stream waits.  The graphs hold kernel launches and event dependencies
+// only.
+struct RaggedGraph {
+  uint64_t key = 0;
+  int dev = -1;
+  hipGraphExec_t exec = nullptr;
+};
+constexpr int kGraphCache = 64;
+
+uint64_t mix64(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h * 0xFF51AFD7ED558CCDull;
+}
+
+template <typename Real>
+int run_ragged_graph(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t* off, const int64_t* off_dev,
+                     int64_t d, long long mcs, int ms, double eps, int32_t* labels, unsigned char* ws,
+                     int32_t* status, int flags) {
+  static std::mutex mu;
+  static RaggedGraph cache[kGraphCache];
+  static int filled = 0;
+  static hipStream_t capture[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail("no current HIP device");
+  uint64_t key = mix64(0, (uint64_t)dev);
+  for (uint64_t v : {(uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)off_dev, (uint64_t)(uintptr_t)labels,
+                     (uint64_t)(uintptr_t)ws, (uint64_t)(uintptr_t)status, (uint64_t)R.windows, (uint64_t)d,
+                     (uint64_t)sizeof(Real), (uint64_t)mcs, (uint64_t)ms, (uint64_t)flags})
+    key = mix64(key, v);
+  uint64_t eb;
+  std::memcpy(&eb, &eps, sizeof(eb));
+  key = mix64(key, eb);
+  for (int64_t w = 0; w <= R.windows; ++w) key = mix64(key, (uint64_t)off[w]);
+  std::lock_guard<std::mutex> lock(mu);
+  for (int i = 0; i < filled; ++i)
+    if (cache[i].key == key && cache[i].dev == dev) {
+      if (hipGraphLaunch(cache[i].exec, s) != hipSuccess) return fail("hipGraphLaunch failed");
+      return 0;
+    }
+  if (filled == kGraphCache) return run_ragged<Real>(s, R, x, off_dev, d, mcs, ms, eps, labels, ws, status);
+  if (!capture[dev] && hipStreamCreateWithFlags(&capture[dev], hipStreamNonBlocking) != hipSuccess)
+    return fail("cannot create the capture stream");
+  hipStream_t c = capture[dev];
+  if (hipStreamBeginCapture(c, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipGetLastError();
+    return run_ragged<Real>(s, R, x, off_dev, d, mcs, ms, eps, labels, ws, status);
+  }
+  const int err = run_ragged<Real>(c, R, x, off_dev, d, mcs, ms, eps, labels, ws, status);
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(c, &graph);
+  if (err || ce != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (err) return err;
+    return run_ragged<Real>(s, R, x, off_dev, d, mcs, ms, eps, labels, ws, status);
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess || !exec) {
+    (void)hipGetLastError();
+    return run_ragged<Real>(s, R, x, off_dev, d, mcs, ms, eps, labels, ws, status);
+  }
+  cache[filled].key = key;
+  cache[filled].dev = dev;
+  cache[filled].exec = exec;
+  ++filled;
+  if (hipGraphLaunch(exec, s) != hipSuccess) return fail("hipGraphLaunch failed");
+  return 0;
 }
 
 // Workspace of hdb_cluster_noalloc: one array per GPU in the library's own
@@ -1071,8 +1223,10 @@ int hdb_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t
   hipStream_t s = static_cast<hipStream_t>(stream);
   // distance_kernel checks every input coordinate (bit 2) on its diagonal tiles.
   if (dtype == HDB_F32)
-    return launch_distances<float>(s, static_cast<const float*>(x), batch, n, d, static_cast<float*>(out), status);
-  return launch_distances<double>(s, static_cast<const double*>(x), batch, n, d, static_cast<double*>(out), status);
+    return launch_distances<float>(s, static_cast<const float*>(x), dense_launch(batch, n), d,
+                                   static_cast<float*>(out), status);
+  return launch_distances<double>(s, static_cast<const double*>(x), dense_launch(batch, n), d,
+                                  static_cast<double*>(out), status);
 }
 
 int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,
@@ -1130,11 +1284,19 @@ int hdb_cluster_ragged(void* stream, const void* x, const int64_t* offsets, cons
   if (workspace_bytes < R.total || !workspace) return fail("workspace too small");
   hipStream_t s = static_cast<hipStream_t>(stream);
   unsigned char* ws = static_cast<unsigned char*>(workspace);
-  if (dtype == HDB_F32)
-    return run_ragged<float>(s, R, static_cast<const float*>(x), offsets_dev, d, min_cluster_size, min_samples,
-                             epsilon, labels, ws, status);
-  return run_ragged<double>(s, R, static_cast<const double*>(x), offsets_dev, d, min_cluster_size, min_samples,
-                            epsilon, labels, ws, status);
+  const bool graph = !(flags & HDB_FLAG_NO_GRAPH);
+  if (dtype == HDB_F32) {
+    const float* xf = static_cast<const float*>(x);
+    return graph ? run_ragged_graph<float>(s, R, xf, offsets, offsets_dev, d, min_cluster_size, min_samples,
+                                           epsilon, labels, ws, status, flags)
+                 : run_ragged<float>(s, R, xf, offsets_dev, d, min_cluster_size, min_samples, epsilon, labels, ws,
+                                     status);
+  }
+  const double* xd = static_cast<const double*>(x);
+  return graph ? run_ragged_graph<double>(s, R, xd, offsets, offsets_dev, d, min_cluster_size, min_samples,
+                                          epsilon, labels, ws, status, flags)
+               : run_ragged<double>(s, R, xd, offsets_dev, d, min_cluster_size, min_samples, epsilon, labels, ws,
+                                    status);
 }
 
 int hdb_cluster_matrix(void* stream, const void* distances, const void* core, int64_t batch, int64_t n, int dtype,

@@ -53,6 +53,7 @@ extern "C" {
 #define HDB_FLAG_FILTER 128     /* MFMA Gram filter also on the matrix path */
 #define HDB_FLAG_ONE_STEP 65536 /* matrix-path Prim: one vertex per step, no row speculation */
 #define HDB_FLAG_GRAM_GRAPH 262144 /* float32 matrix path: MFMA Gram matrix + interval-key Prim (no exact matrix) */
+#define HDB_FLAG_NO_GRAPH 524288    /* ragged calls: launch directly instead of capturing / replaying a HIP graph */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4; 8 on the speculative
  * matrix path only); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,

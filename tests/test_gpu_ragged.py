@@ -133,7 +133,6 @@ def test_ragged_nonfinite_and_overflow_status():
     ([10, 1, 20], 2, 4, "at least 2"),
     ([10, 5000], 2, 4, "4096"),
     ([10, 4, 20], 5, 4, "min_samples"),
-    ([2048, 1500], 5, 64, "points path"),
 ])
 def test_ragged_rejects(sizes, ms, d, message):
     from hdbscan_amd import _native
@@ -160,3 +159,95 @@ def test_cluster_ragged_wrapper():
     assert all(g.is_cuda and g.dtype == torch.int32 for g in got)
     for g, e in zip(got, expected(x_list, 5, 4, 0.0, "float32")):
         np.testing.assert_array_equal(g.cpu().numpy(), e)
+
+
+MATRIX_SIZES = [40, 300, 1100, 1500, 2048, 2049, 3000, 4096]
+
+
+@pytest.mark.parametrize("precision,d", [("float32", 32), ("float32", 100), ("float64", 16)])
+@pytest.mark.parametrize("mcs,ms,eps", [(10, 5, 0.05), (2, 2, 0.0), (5, 40, 0.0)])
+def test_ragged_matrix_buckets(precision, d, mcs, ms, eps):
+    """Windows whose coordinates do not fit the registers of the points path
+    (float32 above 16 dimensions over 1024 points, float64 above 8) take the
+    materialised matrix per window inside the ragged call."""
+    rng = np.random.default_rng(d + ms)
+    x_list = [blobs(rng, n, d, precision) for n in rng.permutation(MATRIX_SIZES)]
+    got, status = ragged(x_list, mcs, ms, eps, precision)
+    assert status == 0
+    for i, (g, e) in enumerate(zip(got, expected(x_list, mcs, ms, eps, precision))):
+        np.testing.assert_array_equal(g, e, err_msg=f"window {i} n={len(x_list[i])}")
+
+
+class RaggedCall:
+    """One ragged problem with fixed device buffers, so that repeated calls
+    have the same signature and replay the captured graph."""
+
+    def __init__(self, x_list, mcs, ms, eps, precision):
+        torch = G.torch()
+        from hdbscan_amd import _native
+
+        self.lib = _native.lib()
+        self.offsets = np.zeros(len(x_list) + 1, dtype=np.int64)
+        self.offsets[1:] = np.cumsum([len(w) for w in x_list])
+        self.packed = G.dev(np.concatenate(x_list, axis=0))
+        self.args = (len(x_list), int(self.packed.shape[1]), G.code(precision), int(mcs), int(ms), float(eps))
+        self.off_ptr = self.offsets.ctypes.data_as(ctypes.c_void_p)
+        self.ws_bytes = self.lib.hdb_ragged_workspace_bytes(self.off_ptr, len(x_list), self.args[1], self.args[2],
+                                                            int(ms), 0)
+        assert self.ws_bytes > 0
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device="cuda")
+        self.off_dev = torch.from_numpy(self.offsets).cuda()
+        self.labels = torch.empty(int(self.offsets[-1]), dtype=torch.int32, device="cuda")
+        self.status = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+    def run(self, flags=0):
+        from hdbscan_amd import _native
+
+        w, d, code, mcs, ms, eps = self.args
+        self.labels.fill_(-7)
+        self.status.zero_()
+        _native.check(self.lib.hdb_cluster_ragged(G.stream(), G.ptr(self.packed), self.off_ptr, G.ptr(self.off_dev),
+                                                  w, d, code, mcs, ms, eps, G.ptr(self.labels), G.ptr(self.ws),
+                                                  self.ws_bytes, G.ptr(self.status), int(flags)))
+        return G.host(self.labels), int(G.host(self.status)[0])
+
+
+def test_ragged_graph_replays_match_direct_launches():
+    """The first call captures a HIP graph, later calls with the same buffers
+    replay it: every replay, with other device work and synchronisations in
+    between, gives the labels of the directly launched call."""
+    torch = G.torch()
+    from hdbscan_amd import _native
+
+    rng = np.random.default_rng(21)
+    x_list = [blobs(rng, n, 16, "float32") for n in (64, 70, 300, 900, 1100, 2100, 3000, 129)]
+    call = RaggedCall(x_list, 10, 5, 0.05, "float32")
+    direct, st = call.run(_native.FLAG_NO_GRAPH)
+    assert st == 0
+    for e, g in zip(expected(x_list, 10, 5, 0.05, "float32"),
+                    [direct[call.offsets[i]:call.offsets[i + 1]] for i in range(len(x_list))]):
+        np.testing.assert_array_equal(g, e)
+    scratch = torch.randn(1 << 20, device="cuda")
+    for rep in range(5):
+        got, st = call.run()
+        assert st == 0
+        np.testing.assert_array_equal(got, direct, err_msg=f"replay {rep}")
+        scratch.mul_(1.5).add_(1.0)  # unrelated device work between replays
+        torch.cuda.synchronize()
+        if rep == 2:
+            again, st = call.run(_native.FLAG_NO_GRAPH)
+            np.testing.assert_array_equal(again, direct)
+
+
+def test_ragged_graph_on_a_side_stream():
+    torch = G.torch()
+    rng = np.random.default_rng(22)
+    x_list = [blobs(rng, n, 8, "float32") for n in (90, 500, 1300)]
+    call = RaggedCall(x_list, 5, 5, 0.0, "float32")
+    base, _ = call.run()
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            got, st = call.run()
+            assert st == 0
+            np.testing.assert_array_equal(got, base)
