@@ -405,6 +405,16 @@ def bench_ragged(args, spec, torch):
         f"(mean {sizes.mean():.0f}), D={d} {precision}")
     packed, offsets = _rocm.pack_windows(windows, precision)
     threads = min(16, host_cores())
+    # The autotuner's table for this dimension / precision / min_samples,
+    # when one is committed (hdbscan_amd/tuning/).
+    from hdbscan_amd import autotune
+
+    tuning = None
+    path = args.tuning or autotune.table_path(d, precision, ms)
+    if path != "none" and os.path.exists(path):
+        autotune.apply(autotune.load(path))
+        tuning = os.path.relpath(path, ROOT)
+        log(f"tuning table {tuning}")
 
     def call(**kw):
         return _rocm.cluster_ragged_async(packed, offsets, mcs, ms, eps, precision, flags=args.flags, **kw)
@@ -451,7 +461,8 @@ def bench_ragged(args, spec, torch):
         "vs_baseline": None, "dtype": "f32" if precision == "float32" else "f64", "data": "synthetic",
         "config": {"workload": args.workload, "windows": count, "points": int(offsets[-1]), "d": d,
                    "min_cluster_size": mcs, "min_samples": ms, "cluster_selection_epsilon": eps,
-                   "flags": args.flags},
+                   "flags": args.flags, "tuning": tuning,
+                   "launch": "HIP graph replay" if not args.flags & 524288 else "direct launches"},
         "label_exact_windows": exact,
         "timing": {"resident": {k: v for k, v in rep.items() if k != "samples_seconds"}},
         "input_sha256": hashlib.sha256(b"".join(np.ascontiguousarray(w).tobytes() for w in windows)).hexdigest(),
@@ -482,6 +493,9 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (3 launches) and exit; "
                          "used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
+    ap.add_argument("--tuning", default=None,
+                    help="ragged workloads: tuning table (default: the committed one for the workload, if any; "
+                         "'none' for the automatic choice)")
     ap.add_argument("--plant-mismatch", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args(argv)
     if args.gpus != 1:

@@ -20,7 +20,7 @@ FLAG_NO_GRAPH = 524288
 EXPORTED = ("hdb_version", "hdb_last_error", "hdb_profile", "hdb_stage_count", "hdb_stage_name",
             "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_cluster_export", "hdb_distances",
             "hdb_core_distances", "hdb_cluster_matrix", "hdb_ragged_workspace_bytes", "hdb_cluster_ragged",
-            "hdb_cluster_noalloc", "hdb_probe_valu")
+            "hdb_cluster_noalloc", "hdb_probe_valu", "hdb_ragged_set_tuning")
 NOALLOC_BYTES = 64 * 1024 * 1024
 
 _lib = None
@@ -51,6 +51,8 @@ def lib():
         handle.hdb_stage_name.argtypes = [_I]
         handle.hdb_stage_ms.restype = ctypes.c_float
         handle.hdb_stage_ms.argtypes = [_I]
+        handle.hdb_ragged_set_tuning.restype = _I
+        handle.hdb_ragged_set_tuning.argtypes = [_P, _I]
         handle.hdb_probe_valu.restype = _I
         handle.hdb_probe_valu.argtypes = [_P, _I, _I, _P, _P]
         handle.hdb_workspace_bytes.restype = ctypes.c_size_t

@@ -187,13 +187,19 @@ def cluster_ragged_async(packed, offsets, min_cluster_size, min_samples, epsilon
 
 
 def cluster_ragged(windows, min_cluster_size, min_samples=None, cluster_selection_epsilon=0.0,
-                   precision="float32", flags=0):
+                   precision="float32", flags=0, tuning=None):
     """Labels of windows of uneven size in one call: a list of device int32
     label tensors, one per window ([N_i]).  Backend-level: the reference's
-    public API is dense only (reference hdbscan_gpu/_api.py:71-78)."""
+    public API is dense only (reference hdbscan_gpu/_api.py:71-78).
+    ``tuning``: a table of ``hdbscan_amd.autotune`` (dict or JSON path) to
+    install for this and later ragged calls."""
     torch = _device_ready()
     if min_samples is None:
         min_samples = min_cluster_size
+    if tuning is not None:
+        from . import autotune
+
+        autotune.apply(autotune.load(tuning) if isinstance(tuning, str) else tuning)
     packed, offsets = pack_windows(windows, precision)
     labels, status, _, _ = cluster_ragged_async(packed, offsets, min_cluster_size, min_samples,
                                                 cluster_selection_epsilon, precision, flags)

@@ -1264,6 +1264,15 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
                              reinterpret_cast<double*>(ws + P.off_matrix), ws, status, false, true);
 }
 
+int hdb_ragged_set_tuning(const int32_t* bucket_flags, int count) {
+  g_error.clear();
+  if (count < 0 || count > kBuckets || (count && !bucket_flags)) return fail("tuning table: 0..7 bucket flags");
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  for (int k = 0; k < kBuckets; ++k) g_tune[k] = k < count ? (bucket_flags[k] & kTuneMask) : 0;
+  g_tune_set = count > 0;
+  return 0;
+}
+
 size_t hdb_ragged_workspace_bytes(const int64_t* offsets, int64_t windows, int64_t d, int dtype,
                                   int32_t min_samples, int flags) {
   g_error.clear();

@@ -155,10 +155,16 @@ int hdb_core_distances(void *stream, const void *x, int64_t batch, int64_t n,
  * `offsets` is the host copy (planning), `offsets_dev` the same values in
  * device memory (read by the kernels).  Labels are written per point into
  * labels[total], numbered within each window as hdb_cluster numbers them.
- * Windows are grouped by size; the groups run concurrently on internal
- * streams forked from and joined into `stream` (one set of internal streams
- * per caller stream, so ragged calls on different streams may overlap).  Windows above 1024 points
- * need d <= 16 (float32) or d <= 8 (float64) (the points path).
+ * Windows are grouped by size into buckets (up to 64, 128, ..., 4096 points);
+ * the buckets run concurrently on internal streams forked from and joined
+ * into `stream` (one set of internal streams per caller stream, so ragged
+ * calls on different streams may overlap).  Each bucket's Prim and tree
+ * launches are persistent: their workgroups claim the bucket's windows from
+ * a device counter, largest first.  The first call of a signature (buffers,
+ * sizes, parameters, offsets) is captured into a HIP graph that later calls
+ * replay (HDB_FLAG_NO_GRAPH: always launch directly).  Windows whose
+ * coordinates do not fit the points path's registers (float32 above 16
+ * dimensions over 1024 points) take a materialised matrix per window.
  * The reference API is dense only (hdbscan_gpu/_api.py:71-78: one [B, N, D]
  * array); this entry serves uneven windows in one call. */
 size_t hdb_ragged_workspace_bytes(const int64_t *offsets, int64_t windows,
@@ -170,6 +176,14 @@ int hdb_cluster_ragged(void *stream, const void *x, const int64_t *offsets,
                        int32_t min_samples, double epsilon, int32_t *labels,
                        void *workspace, size_t workspace_bytes,
                        int32_t *status, int flags);
+
+/* Tuning table of ragged calls (no counterpart in the reference; written by
+ * the autotuner, hdbscan_amd/autotune.py): for size bucket k (windows of up
+ * to 64, 128, 256, 512, 1024, 2048, 4096 points), bucket_flags[k] replaces
+ * the call's tuning bits (vertices per Prim thread, Gram row sets, repacking
+ * interval, HDB_FLAG_GLOBAL_ARENA, HDB_FLAG_NO_POINT_LDS) for that bucket.
+ * count = 0 clears the table.  Process-wide; results do not depend on it. */
+int hdb_ragged_set_tuning(const int32_t *bucket_flags, int count);
 
 /* Graph and tree stages from a given distance matrix [batch][n][n] and core
  * distances [batch][n] (core may be NULL when unit_core != 0, meaning every

@@ -251,3 +251,34 @@ def test_ragged_graph_on_a_side_stream():
             got, st = call.run()
             assert st == 0
             np.testing.assert_array_equal(got, base)
+
+
+def test_tuning_table_changes_no_label():
+    """Any installed tuning table gives the labels of the automatic choice."""
+    from hdbscan_amd import _native, autotune
+
+    rng = np.random.default_rng(31)
+    x_list = [blobs(rng, n, 16, "float32") for n in (40, 120, 250, 500, 1000, 2000, 3500)]
+    base, st = ragged(x_list, 10, 5, 0.05, "float32")
+    assert st == 0
+    flags = [(4 << 8) | (1 << 14), 2 << 8, (1 << 8) | _native.FLAG_GLOBAL_ARENA, 3 << 14,
+             (3 << 12) | _native.FLAG_NO_POINT_LDS, (2 << 8) | (1 << 12), (4 << 8) | (2 << 14)]
+    try:
+        autotune.apply({"d": 16, "precision": "float32", "min_samples": 5,
+                        "buckets": [{"top": t, "chosen_flags": f} for t, f in zip(autotune.BUCKETS, flags)]})
+        got, st = ragged(x_list, 10, 5, 0.05, "float32")
+    finally:
+        autotune.apply(None)
+    assert st == 0
+    for g, b in zip(got, base):
+        np.testing.assert_array_equal(g, b)
+
+
+def test_autotune_small_run():
+    from hdbscan_amd import autotune
+
+    table = autotune.tune(d=8, windows=6, repeats=1, buckets=(128, 1024), log=lambda *a: None)
+    assert [b["top"] for b in table["buckets"]] == [128, 1024]
+    for b in table["buckets"]:
+        assert b["chosen_ms"] <= b["auto_ms"]
+        assert len(b["measurements"]) >= 5
