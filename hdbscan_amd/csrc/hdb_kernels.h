@@ -442,45 +442,24 @@ struct TreeArgs {
   int32_t* status;
   int32_t* export_ints;  // [B][10][n] or null
   void* export_reals;    // [B][4][n] or null
+  uint32_t* sorted_uv;   // split tree stage: sorted endpoints per point (u | v << 16)
+  void* sorted_w;        // split tree stage: sorted weights per point (Real)
 };
 
-// Tree stages of one window (item of the launch) by one workgroup.
+// Tree stages of one window (item of the launch) by one workgroup: the
+// window's arena, its Prim edges as sort keys, then the NumPy-order sort.
 template <typename Real, bool LDSA>
-__device__ __forceinline__ void tree_window(const TreeArgs& a, int item) {
+__device__ __forceinline__ unsigned char* tree_arena(const TreeArgs& a, int b, int64_t p0) {
   extern __shared__ __align__(16) unsigned char lds[];
-  __shared__ int sh_counters[4];
-  __shared__ long long sh_clk[6];
-  const int tid = threadIdx.x, T = blockDim.x;
-  int64_t p0;
-  int n, b;
-  a.win.get(item, b, p0, n);
-  int32_t* lab = a.labels + (size_t)p0;
-  if (a.wflag[b]) {
-    for (int i = tid; i < n; i += T) lab[i] = -1;
-    return;
-  }
-  unsigned char* base =
-      LDSA ? lds
-           : a.arena + (a.arena_stride ? (size_t)b * a.arena_stride
-                                       : (size_t)p0 * kArenaPerPoint + (size_t)b * kArenaPerWindow);
-  const TreeView<Real> t = tree_view<Real>(base, n);
-  const int32_t* eu = a.edge_u + (size_t)p0;
-  const int32_t* ev = a.edge_v + (size_t)p0;
-  const Real* ew = static_cast<const Real*>(a.edge_w) + (size_t)p0;
-  for (int i = tid; i < n - 1; i += T) {
-    EdgeKey<Real> k;
-    k.w = ew[i];
-    k.idx = i;
-    t.keys[i] = k;
-    t.edge_u[i] = (int16_t)eu[i];
-    t.edge_v[i] = (int16_t)ev[i];
-  }
-  __syncthreads();
-  // Phase timestamps (constant 100 MHz clock) for the diagnostic export.
-  long long clk[6];
-  clk[0] = wall_clock64();
+  return LDSA ? lds
+              : a.arena + (a.arena_stride ? (size_t)b * a.arena_stride
+                                          : (size_t)p0 * kArenaPerPoint + (size_t)b * kArenaPerWindow);
+}
+
+template <typename Real>
+__device__ __forceinline__ void tree_sort(const TreeArgs& a, const TreeView<Real>& t, int n, int* counters) {
   if (a.serial_sort) {
-    if (tid == 0) introsort(t.keys, n - 1, a.sort_depth, t.stack);
+    if (threadIdx.x == 0) introsort(t.keys, n - 1, a.sort_depth, t.stack);
     __syncthreads();
   } else {
     // The lists live in birth..stack, which is free until condensation and
@@ -492,8 +471,21 @@ __device__ __forceinline__ void tree_window(const TreeArgs& a, int item) {
     lists.next = lists.cur + 3 * lists.cap;
     lists.heap = lists.next + 3 * lists.cap;
     lists.small = lists.heap + 2 * lists.cap;
-    parallel_introsort(t.keys, n - 1, a.sort_depth, t.uf, t.comp, lists, sh_counters);
+    parallel_introsort(t.keys, n - 1, a.sort_depth, t.uf, t.comp, lists, counters);
   }
+}
+
+// Linkage, condensation, selection and labels of one window whose sorted
+// keys (weights; idx = the edge's index in edge_u / edge_v) are in the
+// arena.  eu / ev / ew: the window's Prim edges (for the exports only).
+template <typename Real>
+__device__ __forceinline__ void tree_finish(const TreeArgs& a, const TreeView<Real>& t, int n, int b, int64_t p0,
+                                            const int32_t* eu, const int32_t* ev, const Real* ew, int* sh_counters,
+                                            long long* sh_clk, long long clk0) {
+  const int tid = threadIdx.x, T = blockDim.x;
+  int32_t* lab = a.labels + (size_t)p0;
+  long long clk[6];
+  clk[0] = clk0;
   {
     // parent / point_parent / birth are free until condensation.
     int16_t* su = t.parent;
@@ -570,12 +562,117 @@ __device__ __forceinline__ void tree_window(const TreeArgs& a, int item) {
   }
 }
 
+
+template <typename Real, bool LDSA>
+__device__ __forceinline__ void tree_window(const TreeArgs& a, int item) {
+  __shared__ int sh_counters[4];
+  __shared__ long long sh_clk[6];
+  const int tid = threadIdx.x, T = blockDim.x;
+  int64_t p0;
+  int n, b;
+  a.win.get(item, b, p0, n);
+  int32_t* lab = a.labels + (size_t)p0;
+  if (a.wflag[b]) {
+    for (int i = tid; i < n; i += T) lab[i] = -1;
+    return;
+  }
+  const TreeView<Real> t = tree_view<Real>(tree_arena<Real, LDSA>(a, b, p0), n);
+  const int32_t* eu = a.edge_u + (size_t)p0;
+  const int32_t* ev = a.edge_v + (size_t)p0;
+  const Real* ew = static_cast<const Real*>(a.edge_w) + (size_t)p0;
+  for (int i = tid; i < n - 1; i += T) {
+    EdgeKey<Real> k;
+    k.w = ew[i];
+    k.idx = i;
+    t.keys[i] = k;
+    t.edge_u[i] = (int16_t)eu[i];
+    t.edge_v[i] = (int16_t)ev[i];
+  }
+  __syncthreads();
+  // Phase timestamps (constant 100 MHz clock) for the diagnostic export.
+  const long long clk0 = wall_clock64();
+  tree_sort<Real>(a, t, n, sh_counters);
+  tree_finish<Real>(a, t, n, b, p0, eu, ev, ew, sh_counters, sh_clk, clk0);
+}
+
+// Split tree stage, first half (one workgroup per window): the NumPy-order
+// sort of the window's Prim edges, written as the sorted-edge transport --
+// endpoints as two int16 in one int32 (u | v << 16) and the weight, in sorted
+// order, at the window's first point.
+template <typename Real, bool LDSA>
+__device__ __forceinline__ void sort_window(const TreeArgs& a, int item) {
+  __shared__ int sh_counters[4];
+  const int tid = threadIdx.x, T = blockDim.x;
+  int64_t p0;
+  int n, b;
+  a.win.get(item, b, p0, n);
+  if (a.wflag[b]) return;
+  const TreeView<Real> t = tree_view<Real>(tree_arena<Real, LDSA>(a, b, p0), n);
+  const int32_t* eu = a.edge_u + (size_t)p0;
+  const int32_t* ev = a.edge_v + (size_t)p0;
+  const Real* ew = static_cast<const Real*>(a.edge_w) + (size_t)p0;
+  for (int i = tid; i < n - 1; i += T) {
+    EdgeKey<Real> k;
+    k.w = ew[i];
+    k.idx = i;
+    t.keys[i] = k;
+  }
+  __syncthreads();
+  tree_sort<Real>(a, t, n, sh_counters);
+  for (int i = tid; i < n - 1; i += T) {
+    const EdgeKey<Real> k = t.keys[i];
+    a.sorted_uv[p0 + i] = (uint32_t)(uint16_t)eu[k.idx] | ((uint32_t)(uint16_t)ev[k.idx] << 16);
+    static_cast<Real*>(a.sorted_w)[p0 + i] = k.w;
+  }
+}
+
+// Split tree stage, second half: linkage, condensation, selection and labels
+// from the sorted-edge transport.  A workgroup needs only max(64, (n-1)/4)
+// threads here, so small windows run several to a compute unit.
+template <typename Real, bool LDSA>
+__device__ __forceinline__ void hierarchy_window(const TreeArgs& a, int item) {
+  __shared__ int sh_counters[4];
+  __shared__ long long sh_clk[6];
+  const int tid = threadIdx.x, T = blockDim.x;
+  int64_t p0;
+  int n, b;
+  a.win.get(item, b, p0, n);
+  int32_t* lab = a.labels + (size_t)p0;
+  if (a.wflag[b]) {
+    for (int i = tid; i < n; i += T) lab[i] = -1;
+    return;
+  }
+  const TreeView<Real> t = tree_view<Real>(tree_arena<Real, LDSA>(a, b, p0), n);
+  const Real* sw = static_cast<const Real*>(a.sorted_w) + (size_t)p0;
+  for (int i = tid; i < n - 1; i += T) {
+    EdgeKey<Real> k;
+    k.w = sw[i];
+    k.idx = i;
+    t.keys[i] = k;
+    const uint32_t uv = a.sorted_uv[p0 + i];
+    t.edge_u[i] = (int16_t)(uv & 0xFFFFu);
+    t.edge_v[i] = (int16_t)(uv >> 16);
+  }
+  __syncthreads();
+  tree_finish<Real>(a, t, n, b, p0, nullptr, nullptr, nullptr, sh_counters, sh_clk, wall_clock64());
+}
+
 // Tree stages, one window per workgroup or, with a queue, windows claimed
 // one after another.  Lane 0 runs the sequential stages; the workgroup
 // stages inputs and writes labels and exports.
 template <typename Real, bool LDSA>
 __global__ __launch_bounds__(1024) void tree_kernel(TreeArgs a) {
   for_each_item(a.win, [&](int item) { tree_window<Real, LDSA>(a, item); });
+}
+
+template <typename Real, bool LDSA>
+__global__ __launch_bounds__(1024) void sort_edges_kernel(TreeArgs a) {
+  for_each_item(a.win, [&](int item) { sort_window<Real, LDSA>(a, item); });
+}
+
+template <typename Real, bool LDSA>
+__global__ __launch_bounds__(1024) void hierarchy_kernel(TreeArgs a) {
+  for_each_item(a.win, [&](int item) { hierarchy_window<Real, LDSA>(a, item); });
 }
 
 }  // namespace hdb

@@ -130,7 +130,18 @@ struct Plan {
   int repack = 0;  // Prim repacking interval (points path)
   bool gram_graph = false;  // matrix path through the MFMA Gram matrix (hdb_gramgraph.h)
   size_t off_alt = 0;
+  bool split_tree = false;  // tree stage as sort + hierarchy kernels (sorted-edge transport)
+  int sort_threads = 0, hier_threads = 0;
+  size_t off_suv = 0, off_sw = 0;
 };
+
+// Workgroup widths of the split tree stage for windows of up to n points:
+// the sort takes about one lane per two edges, the hierarchy the least the
+// condensation allows (n - 1 <= 4 T), so that small windows share a CU.
+int split_sort_threads(int64_t n) { return (int)std::min<int64_t>(1024, std::max<int64_t>(64, (n / 2 + 63) / 64 * 64)); }
+int split_hier_threads(int64_t n) {
+  return (int)std::min<int64_t>(1024, std::max<int64_t>(64, ((n - 1 + 3) / 4 + 63) / 64 * 64));
+}
 
 // One launch of the pipeline: the kernels' window descriptor, the grid's
 // window count, the points covered and the largest window.
@@ -246,6 +257,7 @@ size_t gram_operand_bytes(int dk, int64_t batch, int64_t n) {
   return (size_t)batch * ((n + 15) / 16) * nq * 64 * 16;
 }
 
+
 int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int flags, Plan* p) {
   if (dtype != HDB_F32 && dtype != HDB_F64) return fail("dtype must be HDB_F32 or HDB_F64");
   if (batch < 1 || d < 1 || n < 2) return fail("X requires nonempty batches/features and at least 2 samples");
@@ -348,6 +360,17 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.off_pb = o; o = align256(o + pieces);
   P.off_nb = o; o = align256(o + (pieces_needed ? 3 * bn * 4 : 0));
   P.off_alt = o; o = align256(o + (P.gram_graph ? bn * 4 : 0));
+  // Split tree stage when windows outnumber the compute units at least two
+  // to one: its narrow hierarchy workgroups then share CUs (B=1024 windows of
+  // 1024 points: tree 0.73 -> 0.27 ms; at B=256 the fused kernel is 10 %
+  // faster).  HDB_FLAG_SPLIT_TREE / HDB_FLAG_FUSED_TREE force either; export
+  // calls keep the fused kernel, whose exports include the sort permutation.
+  P.split_tree = !(flags & HDB_FLAG_FUSED_TREE) &&
+                 ((flags & HDB_FLAG_SPLIT_TREE) || batch >= 2 * (int64_t)device_cus());
+  P.sort_threads = split_sort_threads(n);
+  P.hier_threads = split_hier_threads(n);
+  P.off_suv = o; o = align256(o + (P.split_tree ? bn * 4 : 0));
+  P.off_sw = o; o = align256(o + (P.split_tree ? bn * rs : 0));
   // Prim repacking of the points path: flag bits 14..15 (0: every 128
   // steps, 1: never, 2: every 32, 3: every 64).  128 measured best: equal
   // or faster than 64 at N=1024 and 4 % faster on ragged batches, whose
@@ -662,6 +685,24 @@ int launch_prim(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) 
 
 template <typename Real>
 int launch_tree(hipStream_t s, const Plan& P, int64_t batch, const TreeArgs& a) {
+  if (P.split_tree && !a.export_ints) {
+    if (P.tree_lds) {
+      auto fs = sort_edges_kernel<Real, true>;
+      auto fh = hierarchy_kernel<Real, true>;
+      if (int r = allow_lds((const void*)fs, P.tree_lds_bytes)) return r;
+      if (int r = allow_lds((const void*)fh, P.tree_lds_bytes)) return r;
+      fs<<<(unsigned)batch, P.sort_threads, P.tree_lds_bytes, s>>>(a);
+      if (int r = check_launch("sort_edges_kernel")) return r;
+      stage_mark(s, "sort_edges_kernel");
+      fh<<<(unsigned)batch, P.hier_threads, P.tree_lds_bytes, s>>>(a);
+    } else {
+      sort_edges_kernel<Real, false><<<(unsigned)batch, P.sort_threads, 0, s>>>(a);
+      if (int r = check_launch("sort_edges_kernel")) return r;
+      stage_mark(s, "sort_edges_kernel");
+      hierarchy_kernel<Real, false><<<(unsigned)batch, P.hier_threads, 0, s>>>(a);
+    }
+    return check_launch("hierarchy_kernel");
+  }
   if (P.tree_lds) {
     auto fn = tree_kernel<Real, true>;
     if (int r = allow_lds((const void*)fn, P.tree_lds_bytes)) return r;
@@ -748,8 +789,12 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   ta.status = status;
   ta.export_ints = exp_i;
   ta.export_reals = exp_r;
-  ta.win.queue = L.tree_queue;
-  if (int r = launch_tree<Real>(s, P, L.tree_queue ? std::min(L.count, L.persist) : L.count, ta)) return r;
+  ta.sorted_uv = reinterpret_cast<uint32_t*>(ws + P.off_suv);
+  ta.sorted_w = ws + P.off_sw;
+  // The split tree stage runs two kernels over the windows: one item per
+  // workgroup (a claim counter serves one launch).
+  ta.win.queue = P.split_tree ? nullptr : L.tree_queue;
+  if (int r = launch_tree<Real>(s, P, ta.win.queue ? std::min(L.count, L.persist) : L.count, ta)) return r;
   stage_mark(s, "tree_kernel");
   return 0;
 }
@@ -781,7 +826,7 @@ struct RaggedPlan {
   int64_t matrix_elems = 0;
   size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0;
   size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, off_pack = 0;
-  size_t off_list = 0, off_moff = 0, off_queue = 0, off_matrix = 0, total = 0;
+  size_t off_list = 0, off_moff = 0, off_queue = 0, off_matrix = 0, off_suv = 0, off_sw = 0, total = 0;
 };
 
 int bucket_of(int64_t n) {
@@ -866,6 +911,10 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
   R.off_pb = o; o = align256(o + pieces);
   R.off_nb = o; o = align256(o + (R.gram ? 3 * pts * 4 : 0));
   R.off_pack = o; o = align256(o + pts * (rs == 4 ? sizeof(PackRec<float>) : sizeof(PackRec<double>)));
+  bool split = false;
+  for (int k = 0; k < kBuckets; ++k) split |= R.used[k] && R.plan[k].split_tree;
+  R.off_suv = o; o = align256(o + (split ? pts * 4 : 0));
+  R.off_sw = o; o = align256(o + (split ? pts * rs : 0));
   R.off_list = o; o = align256(o + (size_t)windows * 4);
   R.off_moff = o; o = align256(o + (size_t)windows * 8);
   R.off_queue = o; o = align256(o + 2 * kBuckets * 4);
@@ -878,6 +927,7 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
     P.off_core = R.off_core; P.off_arena = R.off_arena; P.off_slots = R.off_slots;
     P.off_cand = R.off_cand; P.off_ncand = R.off_ncand; P.off_pa = R.off_pa; P.off_pb = R.off_pb;
     P.off_nb = R.off_nb; P.off_matrix = R.off_matrix; P.total = R.total; P.off_pack = R.off_pack;
+    P.off_suv = R.off_suv; P.off_sw = R.off_sw;
     P.arena_stride = 0;
     P.slot_stride = 0;
   }

@@ -54,6 +54,8 @@ extern "C" {
 #define HDB_FLAG_ONE_STEP 65536 /* matrix-path Prim: one vertex per step, no row speculation */
 #define HDB_FLAG_GRAM_GRAPH 262144 /* float32 matrix path: MFMA Gram matrix + interval-key Prim (no exact matrix) */
 #define HDB_FLAG_NO_GRAPH 524288    /* ragged calls: launch directly instead of capturing / replaying a HIP graph */
+#define HDB_FLAG_SPLIT_TREE 1048576 /* tree stage as a sort kernel + a narrow hierarchy kernel (sorted-edge transport) */
+#define HDB_FLAG_FUSED_TREE 2097152 /* one tree kernel per window even when windows outnumber the CUs */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4; 8 on the speculative
  * matrix path only); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,

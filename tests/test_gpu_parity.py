@@ -7,7 +7,7 @@ import pytest
 import gpu_helpers as G
 from golden_io import records, same_partition, workload_windows
 from oracle import oracle
-from hdbscan_amd._native import FLAG_GRAM_GRAPH, FLAG_ONE_STEP
+from hdbscan_amd._native import FLAG_GRAM_GRAPH, FLAG_ONE_STEP, FLAG_SPLIT_TREE
 
 pytestmark = pytest.mark.gpu
 
@@ -249,7 +249,8 @@ def test_workload_windows(index):
                                    2 | FLAG_ONE_STEP, 2 | 4 << 8, 2 | 8 << 8,
                                    FLAG_ONE_STEP, 1 << 14, 2 << 14, 3 << 14, FLAG_ONE_STEP | 2 << 14,
                                    1 << 8 | 2 << 14, 4 << 8 | 3 << 14, 2 | FLAG_GRAM_GRAPH,
-                                   2 | FLAG_GRAM_GRAPH | 4 << 8])
+                                   2 | FLAG_GRAM_GRAPH | 4 << 8, FLAG_SPLIT_TREE, FLAG_SPLIT_TREE | 1,
+                                   FLAG_SPLIT_TREE | 16, FLAG_SPLIT_TREE | 2])
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_path_variants_agree(flags, precision):
     rng = np.random.default_rng(11)
@@ -376,3 +377,22 @@ def test_matrix_path_core_selection_edges(precision, ms, flags):
         np.testing.assert_array_equal(ex[0]["edge_w"].view(BITS[precision]), w.view(BITS[precision]), err_msg=name)
         np.testing.assert_array_equal(ex[0]["edge_u"], u, err_msg=name)
         np.testing.assert_array_equal(labels[0], t["labels"], err_msg=name)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("n", [2, 3, 5, 17, 64, 65, 200, 257, 1024, 1025, 2500, 4096])
+def test_split_tree_stage_sizes(precision, n):
+    """The split tree stage (sort kernel, sorted-edge transport, hierarchy
+    kernel of max(64, (n-1)/4) threads) across window sizes: labels of the
+    fused tree kernel and of the oracle."""
+    rng = np.random.default_rng(n)
+    x = (rng.integers(-12, 13, (3, n, 3)) / 4).astype(precision)
+    ms = min(4, n)
+    fused, st = G.cluster(x, 3, ms, 0.0, precision)
+    assert st == 0
+    split, st = G.cluster(x, 3, ms, 0.0, precision, flags=FLAG_SPLIT_TREE)
+    assert st == 0
+    np.testing.assert_array_equal(split, fused)
+    for b in range(3):
+        exp, _ = oracle.hdbscan(x[b], 3, ms, 0.0, precision=precision)
+        np.testing.assert_array_equal(split[b], exp)

@@ -106,7 +106,7 @@ def test_ragged_workload_sample():
         np.testing.assert_array_equal(g, e, err_msg=f"window {i} n={len(x_list[i])}")
 
 
-@pytest.mark.parametrize("flags", [64, 1, 32, 16])
+@pytest.mark.parametrize("flags", [64, 1, 32, 16, 1 << 20, (1 << 20) | 1, (1 << 20) | (1 << 19)])
 def test_ragged_path_flags(flags):
     rng = np.random.default_rng(flags)
     x_list = [blobs(rng, n, 16, "float32") for n in (70, 300, 1200, 2500, 130)]
