@@ -434,6 +434,7 @@ struct TreeArgs {
   double epsilon;
   int sort_depth;
   int serial_sort;       // one-lane introsort instead of the workgroup-parallel one
+  int serial_link;       // linkage by one wave in sort order instead of the parallel run form
   int32_t* labels;
   unsigned char* arena;  // global arena (when not in LDS)
   size_t arena_stride;   // dense: bytes per window; 0: ragged addressing
@@ -487,18 +488,21 @@ __device__ __forceinline__ void tree_finish(const TreeArgs& a, const TreeView<Re
   long long clk[6];
   clk[0] = clk0;
   {
-    // parent / point_parent / birth are free until condensation.
-    int16_t* su = t.parent;
-    int16_t* sv = t.pparent;
-    uint32_t* rec = reinterpret_cast<uint32_t*>(t.birth);
     clk[1] = wall_clock64();
-    linkage_gather(t, n, su, sv, rec);
-    __syncthreads();
-    if (tid < 64) {
+    if (a.serial_link) {
+      // parent / point_parent / birth are free until condensation.
+      int16_t* su = t.parent;
+      int16_t* sv = t.pparent;
+      uint32_t* rec = reinterpret_cast<uint32_t*>(t.birth);
+      linkage_gather(t, n, su, sv, rec);
+      __syncthreads();
       clk[2] = wall_clock64();
-      linkage_wave(t, n, su, sv, rec);
-      clk[3] = wall_clock64();
+      if (tid < 64) linkage_wave(t, n, su, sv, rec);
+    } else {
+      clk[2] = wall_clock64();
+      linkage_parallel(t, n);
     }
+    clk[3] = wall_clock64();
   }
   __syncthreads();
   int32_t* xi = a.export_ints ? a.export_ints + (size_t)b * 10 * n : nullptr;
@@ -643,15 +647,22 @@ __device__ __forceinline__ void hierarchy_window(const TreeArgs& a, int item) {
     return;
   }
   const TreeView<Real> t = tree_view<Real>(tree_arena<Real, LDSA>(a, b, p0), n);
+  // The linkage works on the Prim order: the step of each sorted edge is the
+  // position of its added vertex in the window's Prim output, minus one.
   const Real* sw = static_cast<const Real*>(a.sorted_w) + (size_t)p0;
+  int16_t* step_of = t.uf;
+  for (int s = tid; s < n - 1; s += T) step_of[a.edge_v[p0 + s]] = (int16_t)s;
+  __syncthreads();
   for (int i = tid; i < n - 1; i += T) {
+    const uint32_t uv = a.sorted_uv[p0 + i];
+    const int v = (int)(uv >> 16);
+    const int s = step_of[v];
     EdgeKey<Real> k;
     k.w = sw[i];
-    k.idx = i;
+    k.idx = s;
     t.keys[i] = k;
-    const uint32_t uv = a.sorted_uv[p0 + i];
-    t.edge_u[i] = (int16_t)(uv & 0xFFFFu);
-    t.edge_v[i] = (int16_t)(uv >> 16);
+    t.edge_u[s] = (int16_t)(uv & 0xFFFFu);
+    t.edge_v[s] = (int16_t)v;
   }
   __syncthreads();
   tree_finish<Real>(a, t, n, b, p0, nullptr, nullptr, nullptr, sh_counters, sh_clk, wall_clock64());

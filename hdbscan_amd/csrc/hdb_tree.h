@@ -537,6 +537,188 @@ __device__ void linkage_wave(const TreeView<Real>& t, int n, const int16_t* su, 
   }
 }
 
+// Linkage in parallel, from the Prim order.  Prim visits every connected
+// component of {MST edges of weight < w} (and of weight <= w) as one run of
+// consecutive steps: while such a component has an unvisited vertex, some
+// unvisited vertex of it has a key below (or at) w, and no vertex outside it
+// has.  Position 0 holds vertex 0 and position p >= 1 the vertex added at step
+// p - 1; gap p (between positions p - 1 and p) is that step's edge.  Before the
+// edge of sort rank i (weight w) is applied, the component of each endpoint is
+// therefore a run of positions bounded by gaps of weight >= w -- gaps whose
+// rank is >= r0, the first rank of weight w -- and the newest node of a run of
+// more than one position is n + the largest rank inside it, so:
+//  * an edge whose weight no other edge has (gap g) joins the run ending at
+//    position g - 1 and the run starting at g;
+//  * the edges of one weight are applied in rank order over the runs of that
+//    threshold with a small union-find (one lane per weight): the run of the
+//    added vertex starts at its gap g, the run of the source vertex is found
+//    from the source's position.  A run that starts at no gap of the group is
+//    the leftmost run of its component at weight <= w and is followed by a gap
+//    of the group; that gap's rank names its union-find slot.
+// The run queries go through a max tree of gap ranks (leaves: gap p at P + p;
+// gap 0 and the padding are sentinels above every rank).  Results are the
+// sequential Kruskal's left/right/sizes exactly (reference
+// hdbscan_gpu/_kernels.py:518-556).
+constexpr int16_t kRankSentinel = 32767;
+
+// Nearest gap p <= q whose rank is >= r (gap 0 is a sentinel: always found).
+__device__ __forceinline__ int rank_left(const int16_t* tr, int P, int q, int r) {
+  int node = P + q;
+  if (tr[node] >= r) return q;
+  while (node > 1) {
+    if ((node & 1) && tr[node - 1] >= r) {
+      node -= 1;
+      while (node < P) node = tr[2 * node + 1] >= r ? 2 * node + 1 : 2 * node;
+      return node - P;
+    }
+    node >>= 1;
+  }
+  return 0;
+}
+
+// Nearest gap p > q whose rank is >= r, or n.
+__device__ __forceinline__ int rank_right(const int16_t* tr, int P, int q, int r, int n) {
+  int node = P + q;
+  while (node > 1) {
+    if (!(node & 1) && tr[node + 1] >= r) {
+      node += 1;
+      while (node < P) node = tr[2 * node] >= r ? 2 * node : 2 * node + 1;
+      return min(node - P, n);
+    }
+    node >>= 1;
+  }
+  return n;
+}
+
+// Largest rank over gaps lo..hi (lo <= hi).
+__device__ __forceinline__ int rank_max(const int16_t* tr, int P, int lo, int hi) {
+  int l = lo + P, r = hi + P + 1, res = -1;
+  while (l < r) {
+    if (l & 1) res = max(res, (int)tr[l++]);
+    if (r & 1) res = max(res, (int)tr[--r]);
+    l >>= 1;
+    r >>= 1;
+  }
+  return res;
+}
+
+template <typename Real>
+__device__ void linkage_parallel(const TreeView<Real>& t, int n) {
+  const int tid = threadIdx.x, T = blockDim.x, m = n - 1;
+  if (m <= 0) return;
+  int P = 1;
+  while (P < n) P <<= 1;
+  // Arrays free until condensation: uf (positions, then union-find parents of
+  // 2n slots over uf|comp), birth..stab (the rank tree, 2P int16 <= 8n bytes,
+  // then the slot records, 2n uint32), parent / pparent (tied edges: the run
+  // of the added vertex), left / right / sizes (tied edges: the run of the
+  // source and its slot, until the union-find pass writes the results).
+  int16_t* pos = t.uf;
+  int16_t* tr = reinterpret_cast<int16_t*>(t.birth);
+  if (tid == 0) pos[0] = 0;
+  for (int i = tid; i < P; i += T) {
+    if (i < m) {
+      pos[t.edge_v[i]] = (int16_t)(i + 1);
+      tr[P + t.keys[i].idx + 1] = (int16_t)i;
+    }
+    if (i == 0 || i >= n) tr[P + i] = kRankSentinel;
+  }
+  __syncthreads();
+  for (int half = P >> 1; half >= 1; half >>= 1) {
+    for (int i = half + tid; i < 2 * half; i += T) tr[i] = max(tr[2 * i], tr[2 * i + 1]);
+    __syncthreads();
+  }
+  // The newest node of the run of positions s..e.
+  auto run_node = [&](int s, int e) -> int {
+    return s == e ? (s == 0 ? 0 : (int)t.edge_v[s - 1]) : n + rank_max(tr, P, s + 1, e);
+  };
+  for (int i = tid; i < m; i += T) {
+    const Real w = t.keys[i].w;
+    const bool tie_lo = i > 0 && t.keys[i - 1].w == w;
+    const bool tie_hi = i + 1 < m && t.keys[i + 1].w == w;
+    int r0 = i, r1 = i + 1;
+    if (tie_lo) {
+      int lo = 0, hi = i;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (t.keys[mid].w < w) lo = mid + 1; else hi = mid;
+      }
+      r0 = lo;
+    }
+    if (tie_hi) {
+      int lo = i + 1, hi = m;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (t.keys[mid].w > w) hi = mid; else lo = mid + 1;
+      }
+      r1 = lo;
+    }
+    const int g = t.keys[i].idx + 1;
+    const int ec = rank_right(tr, P, g, r0, n) - 1;  // the added vertex's run: g..ec
+    if (r1 - r0 == 1) {
+      const int sa = rank_left(tr, P, g - 1, r0);    // the source's run: sa..g-1
+      t.left[i] = (int16_t)run_node(sa, g - 1);
+      t.right[i] = (int16_t)run_node(g, ec);
+      t.sizes[i] = (int16_t)(ec + 1 - sa);
+    } else {
+      const int q = pos[t.edge_u[g - 1]];
+      const int sa = rank_left(tr, P, q, r0);
+      const int ea = rank_right(tr, P, q, r0, n) - 1;
+      t.parent[i] = (int16_t)run_node(g, ec);
+      t.pparent[i] = (int16_t)(ec - g + 1);
+      t.left[i] = (int16_t)run_node(sa, ea);
+      t.right[i] = (int16_t)(ea - sa + 1);
+      const int rs = tr[P + sa];
+      t.sizes[i] = (int16_t)(sa >= 1 && rs >= r0 && rs < r1 ? rs : n + tr[P + ea + 1]);
+    }
+  }
+  __syncthreads();
+  // One lane per weight shared by several edges: slots 0..m-1 are the runs
+  // starting at the group's gaps (by rank), n + k the leftmost run before the
+  // gap of rank k; records hold newest node | size << 16.
+  int16_t* par = t.uf;
+  uint32_t* rec = reinterpret_cast<uint32_t*>(t.birth);
+  for (int i = tid; i < m; i += T) {
+    const Real w = t.keys[i].w;
+    if ((i > 0 && t.keys[i - 1].w == w) || !(i + 1 < m && t.keys[i + 1].w == w)) continue;
+    int r1 = i + 2;
+    while (r1 < m && t.keys[r1].w == w) ++r1;
+    for (int k = i; k < r1; ++k) {
+      par[k] = (int16_t)k;
+      rec[k] = (uint32_t)(uint16_t)t.parent[k] | ((uint32_t)(uint16_t)t.pparent[k] << 16);
+      const int sl = t.sizes[k];
+      if (sl >= n) {
+        par[sl] = (int16_t)sl;
+        rec[sl] = (uint32_t)(uint16_t)t.left[k] | ((uint32_t)(uint16_t)t.right[k] << 16);
+      }
+    }
+    for (int k = i; k < r1; ++k) {
+      int fa = t.sizes[k], fc = k;
+      while (par[fa] != fa) {
+        par[fa] = par[par[fa]];
+        fa = par[fa];
+      }
+      while (par[fc] != fc) {
+        par[fc] = par[par[fc]];
+        fc = par[fc];
+      }
+      const uint32_t ka = rec[fa], kc = rec[fc];
+      const uint32_t size = (ka >> 16) + (kc >> 16);
+      t.left[k] = (int16_t)(ka & 0xFFFFu);
+      t.right[k] = (int16_t)(kc & 0xFFFFu);
+      t.sizes[k] = (int16_t)size;
+      const uint32_t merged = (uint32_t)(n + k) | (size << 16);
+      if ((ka >> 16) < (kc >> 16)) {
+        par[fa] = (int16_t)fc;
+        rec[fc] = merged;
+      } else {
+        par[fc] = (int16_t)fa;
+        rec[fa] = merged;
+      }
+    }
+  }
+}
+
 // Condensation, prepared in parallel.  After linkage the sort index of each
 // key is dead; for node n+i:
 //   keys[i].w   = lambda = 1/distance (+inf for distance 0)

@@ -781,6 +781,7 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
   ta.epsilon = eps;
   ta.sort_depth = sort_depth;
   ta.serial_sort = (P.flags & HDB_FLAG_SERIAL_SORT) ? 1 : 0;
+  ta.serial_link = (P.flags & HDB_FLAG_SERIAL_LINKAGE) ? 1 : 0;
   ta.labels = labels;
   ta.arena = ws + P.off_arena;
   ta.arena_stride = P.arena_stride;

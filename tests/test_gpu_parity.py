@@ -7,7 +7,7 @@ import pytest
 import gpu_helpers as G
 from golden_io import records, same_partition, workload_windows
 from oracle import oracle
-from hdbscan_amd._native import FLAG_GRAM_GRAPH, FLAG_ONE_STEP, FLAG_SPLIT_TREE
+from hdbscan_amd._native import FLAG_GRAM_GRAPH, FLAG_ONE_STEP, FLAG_SERIAL_LINKAGE, FLAG_SPLIT_TREE
 
 pytestmark = pytest.mark.gpu
 
@@ -186,6 +186,45 @@ def test_matrix_pipeline_matches_oracle(precision, size, vpt, one_step):
         np.testing.assert_array_equal(labels[b], t["labels"])
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("size", [2, 3, 100, 1025, 4096])
+@pytest.mark.parametrize("flags", [0, FLAG_SERIAL_LINKAGE, 1, 1 | FLAG_SERIAL_LINKAGE])
+def test_linkage_forms_match_oracle(precision, size, flags):
+    """The run-form linkage (default) and the one-wave form, LDS and device
+    arena, on all-equal (one tie group of every edge), retained-edge and
+    quantized matrices: left / right / sizes exactly the oracle's."""
+    mats, core = prim_matrices(size, precision)
+    labels, status, ex = G.cluster_matrix(mats, core, 2, 0.0, precision, flags=flags)
+    assert status == 0
+    for b in range(3):
+        u, v, w = oracle.prim(mats[b], core[b], precision=precision)
+        t = oracle.tree(u, v, w, 2, 0.0, precision=precision)
+        for k in ("order", "left", "right", "sizes"):
+            np.testing.assert_array_equal(ex[b][k], t[k], err_msg=k)
+        np.testing.assert_array_equal(labels[b], t["labels"])
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("levels", [2, 5, 40])
+def test_linkage_ties_from_points(precision, levels):
+    """Mutual-reachability weights with many equal values (points on a coarse
+    grid, duplicates): the weight groups of the run-form linkage."""
+    rng = np.random.default_rng(levels)
+    x = (rng.integers(0, levels, (3, 600, 3)) / levels).astype(precision)
+    x[:, 500:] = x[:, :100]
+    for ms in (2, 4):
+        labels, status, ex = G.cluster_export(x, 3, ms, 0.0, precision)
+        assert status == 0
+        for b in range(3):
+            d = oracle.distances(x[b], precision)
+            c = oracle.core(d, ms, precision)
+            u, v, w = oracle.prim(d, c, precision=precision)
+            t = oracle.tree(u, v, w, 3, 0.0, precision=precision)
+            for k in ("order", "left", "right", "sizes"):
+                np.testing.assert_array_equal(ex[b][k], t[k], err_msg=k)
+            np.testing.assert_array_equal(labels[b], t["labels"])
+
+
 @pytest.mark.parametrize("size", [32, 128])
 @pytest.mark.parametrize("within_pair", [0.0, 0.25])
 @pytest.mark.parametrize("epsilon", [0.0, 0.1, 2.0])
@@ -250,7 +289,8 @@ def test_workload_windows(index):
                                    FLAG_ONE_STEP, 1 << 14, 2 << 14, 3 << 14, FLAG_ONE_STEP | 2 << 14,
                                    1 << 8 | 2 << 14, 4 << 8 | 3 << 14, 2 | FLAG_GRAM_GRAPH,
                                    2 | FLAG_GRAM_GRAPH | 4 << 8, FLAG_SPLIT_TREE, FLAG_SPLIT_TREE | 1,
-                                   FLAG_SPLIT_TREE | 16, FLAG_SPLIT_TREE | 2])
+                                   FLAG_SPLIT_TREE | 16, FLAG_SPLIT_TREE | 2, FLAG_SERIAL_LINKAGE,
+                                   FLAG_SERIAL_LINKAGE | 1, FLAG_SPLIT_TREE | FLAG_SERIAL_LINKAGE])
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_path_variants_agree(flags, precision):
     rng = np.random.default_rng(11)
