@@ -128,28 +128,53 @@ struct RaggedLists {
   int32_t queues;
 };
 
+// One block per 64 windows (one per lane); the batch's window sizes pass
+// through LDS in tiles of 256 (size << 3 | bucket: sizes are at most 4096)
+// and the four waves split each tile, their partial ranks and offsets summed
+// at the end.  Grid: max(ceil(windows / 64), 1) blocks of 256 threads.
 __global__ __launch_bounds__(256) void ragged_lists_kernel(const int64_t* off, int windows, RaggedLists rl,
                                                            int32_t* lists, int64_t* moff, int32_t* queue) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g < rl.queues) queue[g] = 0;
-  if (g >= windows) return;
+  __shared__ uint32_t tile[256];
+  __shared__ int part_rank[4][64];
+  __shared__ int64_t part_mo[4][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (blockIdx.x == 0 && tid < rl.queues) queue[tid] = 0;
   auto bucket = [&](int64_t nv) {
     int k = 0;
     while (k < 7 && nv > rl.top[k]) ++k;
     return k;
   };
-  const int64_t n = off[g + 1] - off[g];
-  const int k = bucket(n);
+  const int g = blockIdx.x * 64 + lane;
+  const int n = g < windows ? (int)(off[g + 1] - off[g]) : 0;
+  const int k = g < windows ? bucket(n) : 8;
   int rank = 0;
   int64_t mo = 0;
-  for (int v = 0; v < windows; ++v) {
-    const int64_t nv = off[v + 1] - off[v];
-    const int kv = bucket(nv);
-    rank += kv == k && (nv > n || (nv == n && v < g));
-    if (v < g && ((rl.matrix_mask >> kv) & 1u)) mo += nv * nv;
+  for (int c0 = 0; c0 < windows; c0 += 256) {
+    const int v = c0 + tid;
+    if (v < windows) {
+      const int nv = (int)(off[v + 1] - off[v]);
+      tile[tid] = ((uint32_t)nv << 3) | (uint32_t)bucket(nv);
+    }
+    __syncthreads();
+    const int cnt = min(256, windows - c0);
+#pragma unroll 4
+    for (int i = wave; i < cnt; i += 4) {
+      const uint32_t e = tile[i];
+      const int nv = (int)(e >> 3), kv = (int)(e & 7u), vv = c0 + i;
+      rank += kv == k && (nv > n || (nv == n && vv < g));
+      if (vv < g && ((rl.matrix_mask >> kv) & 1u)) mo += (int64_t)nv * nv;
+    }
+    __syncthreads();
   }
-  lists[rl.start[k] + rank] = g;
-  moff[g] = mo;
+  part_rank[wave][lane] = rank;
+  part_mo[wave][lane] = mo;
+  __syncthreads();
+  if (wave == 0 && g < windows) {
+    rank = part_rank[0][lane] + part_rank[1][lane] + part_rank[2][lane] + part_rank[3][lane];
+    mo = part_mo[0][lane] + part_mo[1][lane] + part_mo[2][lane] + part_mo[3][lane];
+    lists[rl.start[k] + rank] = g;
+    moff[g] = mo;
+  }
 }
 
 // Issue-rate probe for the exact distance's instruction mix: per coordinate

@@ -993,7 +993,7 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
     }
     rl.matrix_mask = R.matrix_mask;
     rl.queues = 2 * kBuckets;
-    const unsigned blocks = (unsigned)((std::max<int64_t>(R.windows, 2 * kBuckets) + 255) / 256);
+    const unsigned blocks = (unsigned)std::max<int64_t>((R.windows + 63) / 64, 1);
     ragged_lists_kernel<<<blocks, 256, 0, s>>>(off_dev, (int)R.windows, rl, lists, moff, queues);
     err = check_launch("ragged_lists_kernel");
   }
