@@ -823,6 +823,8 @@ struct RaggedPlan {
   bool used[kBuckets] = {};
   bool gram = false, global_arena = false;
   int gram_bucket = -1;
+  int top = -1;       // bucket of the largest windows
+  int64_t head = 0;   // its leading windows run as a separate pass (0: one pass)
   uint32_t matrix_mask = 0;
   int64_t matrix_elems = 0;
   size_t off_eu = 0, off_ev = 0, off_ew = 0, off_flag = 0, off_core = 0, off_arena = 0, off_slots = 0;
@@ -863,6 +865,20 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
   }
   for (int k = 0; k < kBuckets; ++k) R.start[k + 1] = R.start[k] + R.count[k];
   R.points = off[windows];
+  // The largest windows' Prim is the critical path: the windows of the top
+  // bucket within 85 % of its largest size get their core distances and Prim
+  // first, the rest of the bucket follows (the lists order windows largest
+  // first, so they are the list's leading entries).
+  for (int k = 0; k < kBuckets; ++k)
+    if (R.used[k]) R.top = k;
+  {
+    const int64_t thr = R.nmax[R.top] * 17 / 20;
+    for (int64_t w = 0; w < windows; ++w) {
+      const int64_t n = off[w + 1] - off[w];
+      R.head += bucket_of(n) == R.top && n >= thr;
+    }
+    if (R.head >= R.count[R.top] || R.nmax[R.top] <= 512) R.head = 0;
+  }
   const int rs = dtype == HDB_F32 ? 4 : 8;
   int tune[kBuckets] = {};
   bool tuned = false;
@@ -918,7 +934,7 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
   R.off_sw = o; o = align256(o + (split ? pts * rs : 0));
   R.off_list = o; o = align256(o + (size_t)windows * 4);
   R.off_moff = o; o = align256(o + (size_t)windows * 8);
-  R.off_queue = o; o = align256(o + 2 * kBuckets * 4);
+  R.off_queue = o; o = align256(o + (2 * kBuckets + 2) * 4);
   R.off_matrix = o; o = align256(o + (size_t)R.matrix_elems * rs);
   R.total = o;
   // Point the bucket plans at the shared buffers (per-point addressing).
@@ -943,7 +959,7 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
 struct SideStreams {
   std::mutex mu;  // held while a call enqueues on this set
   hipStream_t s[kSideStreams];
-  hipEvent_t fork, head, join[kSideStreams];
+  hipEvent_t fork, head, tail, join[kSideStreams];
 };
 
 int side_streams(hipStream_t caller, SideStreams** out) {
@@ -960,7 +976,8 @@ int side_streams(hipStream_t caller, SideStreams** out) {
           hipEventCreateWithFlags(&fresh->join[i], hipEventDisableTiming) != hipSuccess)
         return fail("cannot create internal streams");
     if (hipEventCreateWithFlags(&fresh->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&fresh->head, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&fresh->head, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&fresh->tail, hipEventDisableTiming) != hipSuccess)
       return fail("cannot create events");
     S = fresh;
   }
@@ -992,7 +1009,7 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
       rl.top[k] = k < kBuckets ? kBucketTop[k] : HDB_MAX_SAMPLES;
     }
     rl.matrix_mask = R.matrix_mask;
-    rl.queues = 2 * kBuckets;
+    rl.queues = 2 * kBuckets + 2;
     const unsigned blocks = (unsigned)std::max<int64_t>((R.windows + 63) / 64, 1);
     ragged_lists_kernel<<<blocks, 256, 0, s>>>(off_dev, (int)R.windows, rl, lists, moff, queues);
     err = check_launch("ragged_lists_kernel");
@@ -1006,14 +1023,40 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
   // longest): it gets the first internal stream to itself, and the other
   // buckets, spread over the remaining streams, start once its core
   // distances are done, so that its first stages have the whole device.
+  // With a head (the top bucket's largest windows), the head runs first,
+  // the rest of the top bucket starts after the head's core distances and
+  // the other buckets after the rest's.
   int used = 0;
   bool waited[kSideStreams] = {false, false, false, false};
-  for (int k = kBuckets - 1; k >= 0 && !err; --k) {
+  struct Pass {
+    int k;
+    int64_t first, count;
+    int32_t* queue;
+  };
+  Pass passes[kBuckets + 1];
+  int npass = 0;
+  for (int k = kBuckets - 1; k >= 0; --k) {
     if (!R.used[k]) continue;
-    const int si = used == 0 ? 0 : 1 + (used - 1) % (kSideStreams - 1);
+    if (k == R.top && R.head > 0) {
+      passes[npass++] = {k, 0, R.head, queues + 2 * k};
+      passes[npass++] = {k, R.head, R.count[k] - R.head, queues + 2 * kBuckets};
+    } else {
+      passes[npass++] = {k, 0, R.count[k], queues + 2 * k};
+    }
+  }
+  const bool split_top = R.head > 0;
+  for (int q = 0; q < npass && !err; ++q) {
+    const Pass& pq = passes[q];
+    const int k = pq.k;
+    // Streams: 0 the (head of the) top bucket; with a head, 1 the rest of
+    // the top bucket and 2, 3 the other buckets; else 1..3 the others.
+    const int si = used == 0 ? 0
+                   : split_top ? (used == 1 ? 1 : 2 + (used - 2) % (kSideStreams - 2))
+                               : 1 + (used - 1) % (kSideStreams - 1);
     hipStream_t t = S->s[si];
     if (!waited[si]) {
-      if (hipStreamWaitEvent(t, si == 0 ? S->fork : S->head, 0) != hipSuccess) {
+      hipEvent_t after = si == 0 ? S->fork : (split_top && used >= 2 ? S->tail : S->head);
+      if (hipStreamWaitEvent(t, after, 0) != hipSuccess) {
         err = fail("stream wait failed");
         break;
       }
@@ -1021,19 +1064,20 @@ int run_ragged(hipStream_t s, const RaggedPlan& R, const Real* x, const int64_t*
     }
     Launch L;
     L.win.off = off_dev;
-    L.win.list = lists + R.start[k];
+    L.win.list = lists + R.start[k] + pq.first;
     L.win.moff = moff;
     L.win.queue = nullptr;
     L.win.n = (int)R.nmax[k];
-    L.win.count = (int)R.count[k];
-    L.count = R.count[k];
+    L.win.count = (int)pq.count;
+    L.count = pq.count;
     L.points = R.points;
     L.nmax = R.nmax[k];
-    L.prim_queue = queues + 2 * k;
-    L.tree_queue = queues + 2 * k + 1;
+    L.prim_queue = pq.queue;
+    L.tree_queue = pq.queue + 1;
     L.persist = (int64_t)cus * std::max(1, 1024 / std::max(64, R.plan[k].threads));
+    hipEvent_t core_done = used == 0 ? S->head : (split_top && used == 1 ? S->tail : nullptr);
     err = run_cluster<Real>(t, R.plan[k], x, nullptr, nullptr, L, d, mcs, ms, ms <= 2, eps, -100000, labels, ws,
-                            status, nullptr, nullptr, false, used == 0 ? S->head : nullptr);
+                            status, nullptr, nullptr, false, core_done);
     ++used;
   }
   for (int i = 0; i < kSideStreams; ++i) {
