@@ -20,6 +20,7 @@ FLAG_NO_GRAPH = 524288
 FLAG_SPLIT_TREE = 1048576
 FLAG_FUSED_TREE = 2097152
 FLAG_SERIAL_LINKAGE = 4194304
+FLAG_NO_GRAM_EXACT = 8388608
 EXPORTED = ("hdb_version", "hdb_last_error", "hdb_profile", "hdb_stage_count", "hdb_stage_name",
             "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_cluster_export", "hdb_distances",
             "hdb_core_distances", "hdb_cluster_matrix", "hdb_ragged_workspace_bytes", "hdb_cluster_ragged",

@@ -1,0 +1,796 @@
+// Exact distance matrix through a certified MFMA filter (float32 windows on
+// the materialised-matrix path).
+//
+// The plain matrix path evaluates every pair's exact distance on the VALU
+// (3 operations per coordinate) and Prim then reads rows of that matrix.
+// Prim, however, needs the exact value of a pair only where it can decide a
+// step: almost every pair is far from both of its points' neighbourhoods and
+// enters Prim only as a key that some nearer pair beats.  Here matrix cores
+// bound every pair and only the pairs the bounds cannot dismiss are
+// evaluated exactly:
+//
+//   gx_prep_kernel    squared norms, the input check, and the bf16 pieces
+//                     hi(x) | lo(x) in the order the matrix cores take them;
+//   gx_bound_kernel   per row i, U2_i: a certified upper bound of the R-th
+//                     smallest squared distance of the row (R = max(ms, 8),
+//                     self included), from the hi.hi product over a sample of
+//                     columns (a subset's order statistic bounds the row's);
+//   gx_matrix_kernel  per pair of the upper triangle, the three-product lower
+//                     bound lb <= s of hdb_gram.h.  Pairs with lb <= U2_i or
+//                     lb <= U2_j are "near" and listed; every other pair
+//                     stores minus a lower bound of its distance (the sign
+//                     bit marks it) in both halves of the matrix;
+//   gx_near_kernel    the exact coordinate-order distance of every near pair,
+//                     stored in both halves (so row r holds the same value
+//                     for (r, c) as row c for (c, r)); near pairs with
+//                     s <= U2_i are appended to row i's core list (they
+//                     include the ms nearest);
+//   gx_core_kernel    the core distance: the ms-th smallest of the row
+//                     (self included), from the core list;
+//   prim_gx_kernel    the speculative two-row Prim of hdb_prim.h over these
+//                     rows.  Every unvisited vertex keeps the smallest weight
+//                     seen in the rows read so far -- exact, or a lower bound
+//                     when the last entry that lowered it was a far one (a
+//                     flag bit).  If a step's argmin is flagged, its exact
+//                     key is computed first (resolution, below) and the
+//                     argmin is taken again.
+//
+// Resolution of vertex r: its row holds, for each visited c, the exact
+// distance or a lower bound lbd; for the latter an upper bound follows from
+// lbd and the pair's bound width, so T = the smallest upper weight over
+// visited c bounds r's key from above, and only far pairs whose lower weight
+// is <= T are evaluated exactly.  r's key is the smallest weight over visited
+// c; its source the earliest visited c with that weight -- what the
+// reference's strict-less update (hdbscan_gpu/_kernels.py:215-320) keeps.
+//
+// Why the steps equal the reference's: an unvisited vertex's key b_j is the
+// smallest of the (exact or lower-bound) weights of the entries read for it,
+// each <= its true weight, so the true key K_j >= b_j.  If b_j is not
+// flagged it is an exact weight and every lower bound read was >= it, so
+// K_j = b_j, and its source is the first visited vertex that reached it (an
+// equal weight read later does not replace it, as in the reference).  An
+// unflagged winner of the argmin of (b, vertex) therefore has the smallest
+// true key, lowest index among equals, and the reference's source.
+//
+// Error bounds.  The matrix kernel's lb is hdb_gram.h's lower form: the
+// three split-bf16 products accumulated on the per-point terms
+// t_i = n_i (1 - c) - 2^-110, so lb <= s for the reference's coordinate-order
+// sum s.  Its upper form (1 + c, + 2^-110) would bound s from above, and the
+// two differ by 2 c S + 2^-108 plus at most c S of rounding each, so
+// s <= lb + 4 E with E = c2 S + 2^-107 (c2 = 1.001 c + 4 u, S = n_i + n_j).
+// Pass A uses the hi.hi product alone, which differs from the three-product
+// value by 2 |sum hi lo + lo hi| <= 2^-7 (1 + 2^-6) S: its upper bound adds
+// E1 = (c2 + 0.008) S + 2^-107 to n_i + n_j - 2 hi . hi.  Tiles holding a
+// point whose squared norm exceeds 2^100 store exact distances only (and pass
+// A gives such rows U2 = +inf).  Lower bounds of distances are sqrt(lb) from
+// the hardware square root scaled by 1 - 2^-18 (the instruction's error is a
+// few ulp); resolution's upper bounds square the stored lower bound back with
+// margins 2^-14 and 2^-18, which exceed the roundings involved.
+#pragma once
+
+#include "hdb_common.h"
+#include "hdb_gram.h"
+#include "hdb_gramgraph.h"
+#include "hdb_prim.h"
+
+namespace hdb {
+
+constexpr int kGxTile = 128;       // matrix tile side (one workgroup of 256 threads)
+constexpr int kGxCoreCap = 64;     // core-list entries per point before the row is scanned
+constexpr int kGxNearPerPoint = 32;  // near-pair list capacity per window: 32 entries per point
+constexpr float kGxPassA = 0.008f;                    // hi.hi-only extra relative to S
+constexpr float kGxDown = 0.99999618530273438f;       // 1 - 2^-18
+constexpr float kGxUp = 1.0000038146972656f;          // 1 + 2^-18
+constexpr float kGxSq = 1.00006103515625f;            // 1 + 2^-14
+
+// Operand entry of window w, 16-point block k, K-step q (of 2 * DK / 32: the
+// hi pieces, then the lo pieces) and lane: 8 bf16 of point 16 k + (lane & 15),
+// coordinates 32 q' + 8 (lane >> 4) ... (q' = q mod DK / 32).  A wave loads a
+// block's K-step as one contiguous kilobyte; the same layout serves as the
+// MFMA's A and B operand.
+__host__ __device__ constexpr size_t gx_operand_entries(int dk, int64_t batch, int64_t n) {
+  return (size_t)batch * (size_t)((n + 15) / 16) * (size_t)(2 * dk / 32) * 64;
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup L to XCD
+// L mod 8), each XCD with its own L2.  The kernels that reread a window's
+// operands map the workgroups of window b to XCD b mod 8, in window order:
+// L = 8 (per * (b / 8) + item) + b mod 8 for the item-th of `per`
+// workgroups of window b.  Grid: 8 * ceil(batch / 8) * per workgroups; the
+// ones past the batch exit.
+typedef float gx_f32x4 __attribute__((ext_vector_type(4)));
+
+// Matrix stores (ordinary stores: non-temporal ones measured slower).
+__device__ __forceinline__ void gx_store(float* p, float v) { *p = v; }
+__device__ __forceinline__ void gx_store4(float* p, float a, float b, float c, float d) {
+  *reinterpret_cast<gx_f32x4*>(p) = gx_f32x4{a, b, c, d};
+}
+
+constexpr int kXcds = 8;
+__device__ __forceinline__ bool xcd_item(int64_t L, int64_t batch, int per, int& b, int& item) {
+  const int64_t k = L / kXcds;
+  b = (int)((k / per) * kXcds + L % kXcds);
+  item = (int)(k % per);
+  return b < batch;
+}
+__host__ inline unsigned xcd_grid(int64_t batch, int per) {
+  return (unsigned)(((batch + kXcds - 1) / kXcds) * kXcds * per);
+}
+
+// One workgroup per 16-point block: the block's coordinates are read as
+// contiguous vectors into LDS, one thread per point forms its squared norm
+// (an FMA chain; any order stays within the bound's gamma_D term), and each
+// thread writes whole 16-byte operand entries; the window's near-pair count
+// is zeroed.  Grid: (nblk, batch).
+template <int DK>
+__global__ __launch_bounds__(256) void gx_prep_kernel(const float* x, int n, int d, int nblk, gram_s16x8* op,
+                                                      float* norms, int32_t* lcnt, int32_t* ncnt, int32_t* status) {
+  constexpr int NQ1 = DK / 32;
+  __shared__ float sx[16][DK + 1];
+  const int w = blockIdx.y, k = blockIdx.x, tid = threadIdx.x;
+  if (k == 0 && tid == 0) ncnt[w] = 0;
+  const int p0 = 16 * k, pts = min(16, n - p0);
+  const float* xb = x + ((size_t)w * n + p0) * d;
+  int bad = 0;
+  for (int e = tid; e < 16 * DK; e += 256) {
+    const int r = e / DK, c = e - r * DK;
+    const float v = r < pts && c < d ? xb[(size_t)r * d + c] : 0.f;
+    bad |= !isfinite(v);
+    sx[r][c] = v;
+  }
+  __syncthreads();
+  if (tid < pts) {
+    float sq = 0.f;
+    for (int c = 0; c < d; ++c) sq = fmaf(sx[tid][c], sx[tid][c], sq);
+    norms[(size_t)w * n + p0 + tid] = sq;
+    lcnt[(size_t)w * n + p0 + tid] = 0;
+  }
+  gram_s16x8* ob = op + ((size_t)w * nblk + k) * (2 * NQ1) * 64;
+  for (int e = tid; e < NQ1 * 64; e += 256) {
+    const int q = e >> 6, lane = e & 63;
+    const int r = lane & 15, c0 = 32 * q + 8 * (lane >> 4);
+    gram_s16x8 hv, lv;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const float v = sx[r][c0 + t];
+      const uint32_t h = bf16_bits(v);
+      hv[t] = (short)h;
+      lv[t] = (short)bf16_bits(v - bf16_value(h));
+    }
+    ob[(size_t)q * 64 + lane] = hv;
+    ob[(size_t)(NQ1 + q) * 64 + lane] = lv;
+  }
+  if (__any(bad) && (tid & 63) == 0) atomicOr(status, 2);
+}
+
+// Pass A.  Each wave takes RB 16-row blocks of one window and sweeps the
+// columns [0, cols); every lane keeps, per row it holds, its two smallest
+// upper bounds (one column residue class mod 16).  The R-th smallest of the
+// 16 lanes' lists of a row is >= the R-th smallest over all sampled columns,
+// which is >= the row's R-th smallest exact squared distance (the pair
+// (i, i) enters with an upper bound of its zero; rows outside the sample get
+// their zero offered once).  Norms outside the bounded range give +inf
+// bounds.  Grid: xcd_grid(batch, ceil(n / (64 RB))).
+template <int DK, int RB>
+__global__ __launch_bounds__(256) void gx_bound_kernel(const gram_s16x8* op, const float* norms, int64_t batch,
+                                                       int n, int nblk, int cols, int R, float e1c, float* U2) {
+  constexpr int NQ1 = DK / 32, NQ2 = 2 * NQ1;
+  int b, item;
+  if (!xcd_item(blockIdx.x, batch, (n + 64 * RB - 1) / (64 * RB), b, item)) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int row0 = (item * 4 + wave) * 16 * RB;
+  if (row0 >= n) return;  // uniform per wave; no barrier below
+  const int g = lane >> 4, l16 = lane & 15;
+  const float* nb = norms + (size_t)b * n;
+  gram_bf16x8 av[RB][NQ1];
+  float hi[RB][4], ei[RB][4], lst[RB][4][2];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    const int k = min(row0 / 16 + rb, nblk - 1);
+#pragma unroll
+    for (int q = 0; q < NQ1; ++q)
+      av[rb][q] = __builtin_bit_cast(gram_bf16x8, op[((size_t)(b * nblk + k) * NQ2 + q) * 64 + lane]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = row0 + 16 * rb + 4 * g + t;
+      const float ni = nb[min(i, n - 1)];
+      const bool ok = ni <= kGramNormLimit;
+      hi[rb][t] = ok ? -0.5f * ni : -INFINITY;
+      ei[rb][t] = ok ? e1c * ni : INFINITY;
+      lst[rb][t][0] = (i >= cols && l16 == 0) ? 0.f : INFINITY;
+      lst[rb][t][1] = INFINITY;
+    }
+  }
+  const int cblocks = (cols + 15) / 16;
+  for (int cb = 0; cb < cblocks; ++cb) {
+    const int j = 16 * cb + l16;
+    const float nj = nb[min(j, n - 1)];
+    const bool ok = nj <= kGramNormLimit && j < cols;
+    const float hj = ok ? -0.5f * nj : -INFINITY;
+    const float ej = ok ? e1c * nj + kGramEabs : INFINITY;
+    gram_bf16x8 bv[NQ1];
+#pragma unroll
+    for (int q = 0; q < NQ1; ++q)
+      bv[q] = __builtin_bit_cast(gram_bf16x8, op[((size_t)(b * nblk + cb) * NQ2 + q) * 64 + lane]);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      gram_f32x4 acc;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = hi[rb][t] + hj;
+#pragma unroll
+      for (int q = 0; q < NQ1; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rb][q], bv[q], acc, 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) list_offer<2>(lst[rb][t], fmaf(-2.f, acc[t], ei[rb][t] + ej));
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float u = group16_kth<2>(lst[rb][t], R, lane);
+      const int i = row0 + 16 * rb + 4 * g + t;
+      if (l16 == 0 && i < n) U2[(size_t)b * n + i] = u;
+    }
+}
+
+__device__ __forceinline__ void gx_append(int32_t* lcnt, float* lval, int64_t p, float v) {
+  const int k = atomicAdd(lcnt + p, 1);
+  if (k < kGxCoreCap) lval[(size_t)p * kGxCoreCap + k] = v;
+}
+
+// 128 x 128 tiles of the upper triangle of window b's matrix (tile t of the
+// triangle in row order), 256 threads, each wave a 64 x 64 quarter as 4 x 4
+// MFMA blocks of 16 x 16.  Persistent: workgroup g takes tile ids g, g + G,
+// ... (G workgroups, a multiple of 8, so g keeps its XCD), so that the
+// stores of one tile drain while the next tile's operands load.  Far values are written as soon as they are
+// classified (the tile and its mirror); near pairs go to the wave's LDS list
+// and are evaluated exactly by the whole workgroup afterwards, so no address
+// is written twice.  Registers are held to four waves per SIMD so that the
+// exact evaluations of some tiles overlap the matrix products of others.
+// Tile ids: xcd_grid(batch, nt (nt + 1) / 2), nt = ceil(n / 128).
+template <int DK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void gx_matrix_kernel(const float* x, int64_t batch, int n, int d, int nblk, const gram_s16x8* op,
+                      const float* norms, const float* U2, float* M, int32_t* lcnt, float* lval,
+                      int32_t* ncnt, uint32_t* nlist, int cap, int32_t* status) {
+  constexpr int NQ1 = DK / 32, NQ2 = 2 * NQ1, NQ3 = 3 * NQ1;
+  constexpr float kLower = gram_c(DK);
+  __shared__ float s_n[2][kGxTile];
+  __shared__ float s_u[2][kGxTile];
+  __shared__ uint16_t s_pos[4][kGxTile * kGxTile / 4];
+  const int nt = (n + kGxTile - 1) / kGxTile;
+  const int per = nt * (nt + 1) / 2;
+  const int64_t ids = ((batch + kXcds - 1) / kXcds) * kXcds * per;
+  int bad = 0;
+  for (int64_t L = blockIdx.x; L < ids; L += gridDim.x) {
+    int b, t;
+    if (!xcd_item(L, batch, per, b, t)) continue;  // uniform per workgroup
+    int ti = 0;
+    while (t >= nt - ti) {
+      t -= nt - ti;
+      ++ti;
+    }
+    const int tj = ti + t;
+    const int r0 = ti * kGxTile, c0 = tj * kGxTile;
+    const bool diag = ti == tj;
+    const int64_t base = (int64_t)b * n;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane >> 4, l16 = lane & 15;
+    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    float* mb = M + (size_t)base * n;
+    const float* xb = x + (size_t)base * d;
+    float npt;
+    {
+      const int side = tid >> 7, k = tid & (kGxTile - 1);
+      const int p = (side ? c0 : r0) + k;
+      const bool in = p < n;
+      npt = in ? norms[base + p] : 0.f;
+      // The lower norm term n (1 - c) - 2^-110 of hdb_gram.h: the three
+      // split-bf16 products accumulated on t_i + t_j give a lower bound of
+      // the reference's squared distance.
+      s_n[side][k] = fmaf(npt, -kLower, npt) - kGramEh;
+      s_u[side][k] = in ? U2[base + p] : -INFINITY;
+    }
+    // A point outside the bounded range sends the tile to the exact path.
+    const bool exact_tile = __syncthreads_or(!(npt <= kGramNormLimit));
+    if (!exact_tile) {
+      int cnt = 0;
+      // The lower-left quarter of a diagonal tile holds no pair i < j.
+      if (!(diag && wr > wc)) {
+        gram_f32x4 acc[4][4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              acc[rb][cb][r] = -0.5f * (s_n[0][wr + 16 * rb + 4 * g + r] + s_n[1][wc + 16 * cb + l16]);
+        // Operand blocks of the wave's rows and columns (blocks past the window
+        // are clamped to its last one; their results are not stored).
+        const gram_s16x8* opl = op + lane;
+        size_t aoff[4], boff[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          aoff[k] = (size_t)(b * nblk + min((r0 + wr) / 16 + k, nblk - 1)) * NQ2 * 64;
+          boff[k] = (size_t)(b * nblk + min((c0 + wc) / 16 + k, nblk - 1)) * NQ2 * 64;
+        }
+        // K-steps: A = hi | hi | lo, B = hi | lo | hi (products hi.hi, hi.lo,
+        // lo.hi).  Unrolled by two only: deeper unrolling prefetches operands
+        // past the register budget of four waves per SIMD.
+#pragma unroll 2
+        for (int q = 0; q < NQ3; ++q) {
+          const int seg = q / NQ1, qq = q % NQ1;
+          const int qa = seg == 2 ? NQ1 + qq : qq, qb = seg == 1 ? NQ1 + qq : qq;
+          gram_bf16x8 av[4], bv[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            av[k] = __builtin_bit_cast(gram_bf16x8, opl[aoff[k] + (size_t)qa * 64]);
+            bv[k] = __builtin_bit_cast(gram_bf16x8, opl[boff[k] + (size_t)qb * 64]);
+          }
+#pragma unroll
+          for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rb], bv[cb], acc[rb][cb], 0, 0, 0);
+        }
+        const bool vec = !diag && (n & 3) == 0;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          const int li0 = wr + 16 * rb + 4 * g, i0 = r0 + li0;
+          float ur[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ur[r] = s_u[0][li0 + r];
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            const int lj = wc + 16 * cb + l16, j = c0 + lj;
+            const float uc = s_u[1][lj];
+            float v[4];
+            bool nr[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = i0 + r;
+              const bool valid = (i < n) & (j < n) & (!diag | (i < j));
+              const float lbs = -2.f * acc[rb][cb][r];
+              nr[r] = valid & (lbs <= fmaxf(ur[r], uc));
+              v[r] = lbs > 0.f ? -(__builtin_amdgcn_sqrtf(lbs) * kGxDown) : -0.f;
+              const uint64_t bal = __ballot(nr[r]);
+              if (bal) {
+                const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (nr[r]) s_pos[wave][slot] = (uint16_t)(((li0 + r) << 7) | lj);
+                cnt += __popcll(bal);
+              }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = i0 + r;
+              if (i < n && j < n && (!diag || i <= j) && !nr[r]) gx_store(mb + (size_t)i * n + j, i == j ? 0.f : v[r]);
+            }
+            // Mirror: row j, columns i0 .. i0 + 3.
+            if (j < n) {
+              if (vec && i0 + 3 < n && !(nr[0] || nr[1] || nr[2] || nr[3])) {
+                gx_store4(mb + (size_t)j * n + i0, v[0], v[1], v[2], v[3]);
+              } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  const int i = i0 + r;
+                  if (i < n && (!diag || i < j) && !nr[r]) gx_store(mb + (size_t)j * n + i, v[r]);
+                }
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // The wave's near pairs go to the window's list (one reservation per
+      // wave) for gx_near_kernel; pairs past the list's capacity are
+      // evaluated here.
+      int first = 0;
+      if (lane == 0 && cnt) first = atomicAdd(ncnt + b, cnt);
+      first = __shfl(first, 0);
+      const int room = max(0, min(cnt, cap - first));
+      for (int e = lane; e < cnt; e += 64) {
+        const int pos = s_pos[wave][e];
+        const int li = pos >> 7, lj = pos & 127;
+        const int i = r0 + li, j = c0 + lj;
+        if (e < room) {
+          nlist[(size_t)b * cap + first + e] = ((uint32_t)i << 12) | (uint32_t)j;
+          continue;
+        }
+        const float sq = exact_sq(xb + (size_t)i * d, xb + (size_t)j * d, d);
+        bad |= !isfinite(sq);
+        const float dv = sqrtf(sq);
+        gx_store(mb + (size_t)i * n + j, dv);
+        gx_store(mb + (size_t)j * n + i, dv);
+        if (sq <= s_u[0][li]) gx_append(lcnt, lval, base + i, dv);
+        if (sq <= s_u[1][lj]) gx_append(lcnt, lval, base + j, dv);
+      }
+    } else {
+      // Every pair of the tile exactly (a point beyond the bounded norm range).
+      for (int idx = tid; idx < kGxTile * kGxTile; idx += 256) {
+        const int li = idx >> 7, lj = idx & 127;
+        const int i = r0 + li, j = c0 + lj;
+        if (i >= n || j >= n || (diag && i > j)) continue;
+        if (i == j) {
+          gx_store(mb + (size_t)i * n + j, 0.f);
+          continue;
+        }
+        const float sq = exact_sq(xb + (size_t)i * d, xb + (size_t)j * d, d);
+        bad |= !isfinite(sq);
+        const float dv = sqrtf(sq);
+        gx_store(mb + (size_t)i * n + j, dv);
+        gx_store(mb + (size_t)j * n + i, dv);
+        if (sq <= s_u[0][li]) gx_append(lcnt, lval, base + i, dv);
+        if (sq <= s_u[1][lj]) gx_append(lcnt, lval, base + j, dv);
+      }
+    }
+    __syncthreads();  // the next tile rewrites the LDS arrays
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+}
+
+// Exact distances of the listed near pairs (coordinate order), one lane per
+// pair, written to both halves; core-list appends as in gx_matrix_kernel.
+// One workgroup per (window, chunk of 256 entries), window b on XCD b mod 8
+// and in window order there, so that a window's points stay in that XCD's
+// L2 while its pairs are evaluated; chunks past a window's count exit.
+// Grid: xcd_grid(batch, ceil(cap / 256)).
+__global__ __launch_bounds__(256) void gx_near_kernel(const float* x, int64_t batch, int n, int d, int cap,
+                                                      const int32_t* ncnt, const uint32_t* nlist, const float* U2,
+                                                      float* M, int32_t* lcnt, float* lval, int32_t* status) {
+  int b, chunk;
+  if (!xcd_item(blockIdx.x, batch, (cap + 255) / 256, b, chunk)) return;
+  const int cnt = min(ncnt[b], cap);
+  if (chunk * 256 >= cnt) return;  // uniform per workgroup
+  const int e = chunk * 256 + (int)threadIdx.x;
+  int bad = 0;
+  if (e < cnt) {
+    const uint32_t pe = nlist[(size_t)b * cap + e];
+    const int i = (int)(pe >> 12), j = (int)(pe & 4095u);
+    const int64_t base = (int64_t)b * n;
+    const float sq = exact_sq(x + (size_t)(base + i) * d, x + (size_t)(base + j) * d, d);
+    bad = !isfinite(sq);
+    const float dv = sqrtf(sq);
+    float* mb = M + (size_t)base * n;
+    gx_store(mb + (size_t)i * n + j, dv);
+    gx_store(mb + (size_t)j * n + i, dv);
+    if (sq <= U2[base + i]) gx_append(lcnt, lval, base + i, dv);
+    if (sq <= U2[base + j]) gx_append(lcnt, lval, base + j, dv);
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+}
+
+// Core distance of every point: the ms-th smallest distance of its row, self
+// (+0) included -- the (ms - 1)-th smallest of its core list, which holds
+// every other point at squared distance <= U2 (a superset of the ms - 1
+// nearest).  A list that overflowed: the row's exact entries (sign bit
+// clear) are scanned instead; they include every near pair.  One wave per
+// point; ms in 3 .. K.
+template <int K>
+__global__ __launch_bounds__(256) void gx_core_kernel(const float* M, int64_t points, int n, int ms,
+                                                      const int32_t* lcnt, const float* lval, float* core) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t p = (int64_t)blockIdx.x * 4 + wave;
+  if (p >= points) return;  // uniform per wave
+  const int cnt = lcnt[p];
+  float res;
+  if (cnt <= kGxCoreCap) {
+    float lst[1] = {lane < cnt ? lval[(size_t)p * kGxCoreCap + lane] : INFINITY};
+    res = wave_kth<1>(lst, ms - 1);
+  } else {
+    const float* row = M + (size_t)p * n;
+    float lst[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) lst[q] = INFINITY;
+    for (int j = lane; j < n; j += 64) {
+      const float v = row[j];
+      if (!(__float_as_uint(v) >> 31)) list_offer<K>(lst, v);
+    }
+    res = wave_kth<K>(lst, ms);
+  }
+  if (lane == 0) core[p] = res;
+}
+
+struct GxPrimArgs {
+  const float* m;      // [B][n][n]: exact distance, or -(lower bound) with the sign bit set
+  const float* x;      // points [B][n][d]
+  const float* core;   // per point, null when unit_core
+  const float* norms;  // squared norms per point
+  int n, d, unit_core;
+  float c2;
+  int32_t* edge_u;
+  int32_t* edge_v;
+  float* edge_w;
+  int32_t* wflag;
+  int32_t* status;
+};
+
+// Key (weight, vertex) with the "floor not above the key" flag in the lowest
+// bit of the vertex word: keys order by weight, then vertex.
+__device__ __forceinline__ Cand<float> gx_key(float w, int j, bool flag) {
+  Cand<float> c;
+  c.k = ((uint64_t)__float_as_uint(w) << 32) | ((uint32_t)j << 1) | (flag ? 1u : 0u);
+  return c;
+}
+
+// Workgroup-wide unsigned 64-bit minimum (all threads, uniform result).
+__device__ __forceinline__ uint64_t gx_block_umin64(uint64_t v, uint64_t* sh, int W) {
+  v = wave_umin64(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint64_t r = ~0ull;
+  for (int w = 0; w < W; ++w) r = min(r, sh[w]);
+  return r;
+}
+
+// Relaxation of one row entry v (exact distance, or minus a lower bound) of
+// the row of visited vertex c for an unvisited vertex with key b, source s
+// and flag bit `bit` of fl; m = max of the two core distances (0 with unit
+// cores).  A weight strictly below the key replaces it; the flag records
+// whether it is a lower bound.  (Non-finite distances are reported by the
+// matrix kernel.)
+__device__ __forceinline__ void gx_relax(float v, float m, int c, float& b, int& s, uint32_t& fl, uint32_t bit) {
+  const float w = fmaxf(m, fabsf(v));
+  const bool lt = w < b;
+  b = lt ? w : b;
+  s = lt ? c : s;
+  fl = lt ? (__float_as_int(v) < 0 ? fl | bit : fl & ~bit) : fl;
+}
+
+template <int VPT>
+__global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
+  using C = Cand<float>;
+  const int b = blockIdx.x;
+  const int n = a.n;
+  const int64_t base = (int64_t)b * n;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, W = T >> 6;
+  const bool unit = a.unit_core != 0;
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ C red[2][3][16];
+  __shared__ uint64_t rsh[2][16];
+  __shared__ int sh_err, rc_n;
+  __shared__ int rc_c[64];
+  __shared__ float rc_v[64];
+  float* cs = reinterpret_cast<float*>(lds);
+  const float* mg = a.m + (size_t)base * n;
+  if (tid == 0) sh_err = 0;
+  if (!unit)
+    for (int i = tid; i < n; i += T) cs[i] = a.core[base + i];
+  float best[VPT], cr[VPT];
+  int src[VPT], vst[VPT];
+  uint32_t vis = 0, flg = 0;
+#pragma unroll
+  for (int s = 0; s < VPT; ++s) {
+    const int j = tid + s * T;
+    best[s] = INFINITY;
+    src[s] = 1;
+    vst[s] = 0;
+    cr[s] = j < n && !unit ? a.core[base + j] : 0.f;
+    if (j >= n) vis |= 1u << s;
+  }
+  if (tid == 0) vis |= 1u;  // vertex 0 starts the tree (visit step 0)
+  __syncthreads();
+
+  const float rmax = RealTraits<float>::max_value();
+  const C none = C::make(rmax, 0);
+  int cur = 0, pr = -1, par = 0, bad = 0;
+  bool rowless = false;
+
+  // Exact key and source of unvisited vertex r over the visited vertices.
+  // Far pairs whose lower weight is <= T are listed in LDS and evaluated one
+  // pair per wave (all coordinates loaded at once; entries past the list's
+  // 64 slots by their owner lane).
+  auto resolve = [&](int r) {
+    const float cor = unit ? 0.f : cs[r];
+    const float nrm = a.norms[base + r];
+    const float* mr = mg + (size_t)r * n;
+    if (tid == 0) rc_n = 0;
+    float tmin = INFINITY, wx[VPT], kl[VPT], cm[VPT];
+    uint32_t farm = 0, okm = 0;
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int c = tid + s * T;
+      wx[s] = INFINITY;
+      kl[s] = INFINITY;
+      cm[s] = unit ? 0.f : fmaxf(cr[s], cor);
+      if (((vis >> s) & 1u) && c < n) {
+        okm |= 1u << s;
+        const float v = mr[c];
+        if (__float_as_int(v) < 0) {
+          const float lbd = -v;
+          kl[s] = fmaxf(cm[s], lbd);
+          const float E = a.c2 * (a.norms[base + c] + nrm) + kGramEabs;
+          const float ubs = (lbd * lbd * kGxSq + 4.f * E) * kGxUp;
+          tmin = fminf(tmin, fmaxf(cm[s], __builtin_amdgcn_sqrtf(ubs) * kGxUp));
+          farm |= 1u << s;
+        } else {
+          wx[s] = fmaxf(cm[s], v);
+          tmin = fminf(tmin, wx[s]);
+        }
+      }
+    }
+    const uint64_t tk = gx_block_umin64((uint64_t)__float_as_uint(tmin) << 32, rsh[0], W);
+    const float tb = __uint_as_float((uint32_t)(tk >> 32));
+    int slot[VPT];
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      slot[s] = -1;
+      if (((farm >> s) & 1u) && kl[s] <= tb) {
+        slot[s] = atomicAdd(&rc_n, 1);
+        if (slot[s] < 64) rc_c[slot[s]] = tid + s * T;
+      }
+    }
+    __syncthreads();
+    const int nc = min(rc_n, 64);
+    for (int e = wave; e < nc; e += W) {
+      const float sq = wave_exact_sq1(a.x + (size_t)(base + rc_c[e]) * a.d, a.x + (size_t)(base + r) * a.d, a.d);
+      if (lane == 0) rc_v[e] = sq;
+    }
+    __syncthreads();
+    uint64_t key = ~0ull;
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      if (!((okm >> s) & 1u)) continue;
+      const int c = tid + s * T;
+      float w = wx[s];
+      if ((farm >> s) & 1u) {
+        if (slot[s] < 0) continue;
+        float sq;
+        if (slot[s] < 64) {
+          sq = rc_v[slot[s]];
+        } else {
+          const float* xc = a.x + (size_t)(base + c) * a.d;
+          const float* xr = a.x + (size_t)(base + r) * a.d;
+          sq = 0.f;
+#pragma unroll 1
+          for (int k = 0; k < a.d; ++k) {
+            const float dl = xc[k] - xr[k];
+            sq = sq + dl * dl;
+          }
+        }
+        bad |= !isfinite(sq);
+        w = fmaxf(cm[s], sqrtf(sq));
+      }
+      key = min(key, ((uint64_t)__float_as_uint(w) << 32) | ((uint32_t)vst[s] << 12) | (uint32_t)c);
+    }
+    key = gx_block_umin64(key, rsh[1], W);
+#pragma unroll
+    for (int s = 0; s < VPT; ++s)
+      if (tid + s * T == r) {
+        best[s] = __uint_as_float((uint32_t)(key >> 32));
+        src[s] = (int)(key & 0xFFFu);
+        flg &= ~(1u << s);
+      }
+  };
+
+  for (int step = 0; step < n - 1;) {
+    const bool spec = !rowless && pr >= 0 && n - 1 - step >= 2;
+    const float ccur = rowless || unit ? 0.f : cs[cur];
+    const float cpr = spec && !unit ? cs[pr] : 0.f;
+    // Read conditions of every slot first, then all row loads together, so
+    // that no wait separates the loads of one slot from those of another.
+    float d1[VPT], d2[VPT];
+    bool l1[VPT], l2[VPT];
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      const bool un = !((vis >> s) & 1u);
+      l1[s] = !rowless && un && (unit || fmaxf(ccur, cr[s]) < best[s]);
+      l2[s] = spec && un && j != pr && (unit || fmaxf(cpr, cr[s]) < best[s]);
+    }
+    const float* row1 = mg + (size_t)(rowless ? 0 : cur) * n;
+    const float* row2 = mg + (size_t)(spec ? pr : 0) * n;
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      d1[s] = l1[s] ? row1[tid + s * T] : 0.f;
+      d2[s] = l2[s] ? row2[tid + s * T] : 0.f;
+    }
+    float b1[VPT], b2[VPT];
+    int s1[VPT], s2[VPT];
+    uint32_t fm1 = flg, fm2 = flg;
+    C k1a = none, k1b = none, k2a = none, k2b = none;
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      b1[s] = best[s];
+      s1[s] = src[s];
+      b2[s] = best[s];
+      s2[s] = src[s];
+      if (!((vis >> s) & 1u)) {
+        if (l1[s]) gx_relax(d1[s], unit ? 0.f : fmaxf(ccur, cr[s]), cur, b1[s], s1[s], fm1, 1u << s);
+        const C o1 = gx_key(b1[s], j, (fm1 >> s) & 1u);
+        if (spec)
+          k1a = cand_pick(o1.less(k1a), o1, k1a);
+        else
+          cand_top2(k1a, k1b, o1);
+        if (spec && j != pr) {
+          b2[s] = b1[s];
+          s2[s] = s1[s];
+          fm2 = (fm2 & ~(1u << s)) | (fm1 & (1u << s));
+          if (unit || fmaxf(cpr, cr[s]) < b1[s])
+            gx_relax(d2[s], unit ? 0.f : fmaxf(cpr, cr[s]), pr, b2[s], s2[s], fm2, 1u << s);
+          cand_top2(k2a, k2b, gx_key(b2[s], j, (fm2 >> s) & 1u));
+        }
+      }
+    }
+    const C pa = cand_pick(spec, k2a, k1a), pb = cand_pick(spec, k2b, k1b);
+    const C w1 = wave_min(pa);
+    const C wc = spec ? wave_min(k1a) : w1;
+    const C w2 = wave_min(cand_pick(cand_eq(pa, w1), pb, pa));
+    if (lane == 0) {
+      red[par][0][wave] = wc;
+      red[par][1][wave] = w1;
+      red[par][2][wave] = w2;
+    }
+    __syncthreads();
+    C qc = none, q1 = none, q2 = none;
+    if (lane < W) {
+      qc = red[par][0][lane];
+      q1 = red[par][1][lane];
+      q2 = red[par][2][lane];
+    }
+    par ^= 1;
+    const C g1 = row0_min(q1);
+    const C gc = spec ? row0_min(qc) : g1;
+    const C g2 = row0_min(cand_pick(cand_eq(q1, g1), q2, q1));
+    const C first = cand_pick(spec, gc, g1);
+    const int v1 = first.index() >> 1;
+    const bool fl1 = first.index() & 1;
+    // A winner whose floor is not above its key is not added: this step's
+    // keys are kept (c's row; p's row is dropped) and it is resolved below.
+    const bool hit = !fl1 && spec && v1 == pr;
+    const bool fl2 = hit && (g1.index() & 1);
+    const int v2 = hit && !fl2 ? g1.index() >> 1 : -1;
+#pragma unroll
+    for (int s = 0; s < VPT; ++s) {
+      const int j = tid + s * T;
+      if (j == v1 && !fl1) {
+        if ((vis >> s) & 1u) bad |= 1;
+        vis |= 1u << s;
+        vst[s] = step + 1;
+        const float gw = first.weight();
+        const size_t o = (size_t)base + step;
+        a.edge_u[o] = gw < rmax ? s1[s] : 0;
+        a.edge_v[o] = v1;
+        a.edge_w[o] = gw;
+      }
+      if (j == v2) {
+        if ((vis >> s) & 1u) bad |= 1;
+        vis |= 1u << s;
+        vst[s] = step + 2;
+        const float gw = g1.weight();
+        const size_t o = (size_t)base + step + 1;
+        a.edge_u[o] = gw < rmax ? s2[s] : 0;
+        a.edge_v[o] = v2;
+        a.edge_w[o] = gw;
+      }
+      best[s] = hit ? b2[s] : b1[s];
+      src[s] = hit ? s2[s] : s1[s];
+    }
+    flg = hit ? fm2 : fm1;
+    if (fl1 || fl2) {
+      // The winner (or, after p was added, the runner-up) needs its exact
+      // key before it can be added; the next round reads no row.
+      step += fl2 ? 1 : 0;
+      resolve(fl1 ? v1 : g1.index() >> 1);
+      rowless = true;
+      pr = -1;
+      continue;
+    }
+    step += hit ? 2 : 1;
+    cur = hit ? v2 : v1;
+    rowless = false;
+    pr = (!spec || hit) && g2.less(none) ? g2.index() >> 1 : -1;
+  }
+  if (bad) atomicOr(&sh_err, 1);
+  __syncthreads();
+  if (tid == 0) {
+    a.wflag[b] = sh_err;
+    if (sh_err) atomicOr(a.status, sh_err);
+  }
+}
+
+}  // namespace hdb

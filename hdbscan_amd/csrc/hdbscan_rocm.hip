@@ -26,6 +26,7 @@
 #include "../../include/hdbscan_rocm.h"
 #include "hdb_common.h"
 #include "hdb_gram.h"
+#include "hdb_gramexact.h"
 #include "hdb_gramgraph.h"
 #include "hdb_kernels.h"
 #include "hdb_prim.h"
@@ -129,6 +130,9 @@ struct Plan {
   size_t off_cand = 0, off_ncand = 0, off_pa = 0, off_pb = 0, off_nb = 0, off_pack = 0, total = 0;
   int repack = 0;  // Prim repacking interval (points path)
   bool gram_graph = false;  // matrix path through the MFMA Gram matrix (hdb_gramgraph.h)
+  bool gram_exact = false;  // matrix path with MFMA-bounded near/far entries (hdb_gramexact.h)
+  int gx_dk = 0, gx_cols = 0, gx_rank = 0;
+  size_t off_gop = 0, off_gnorm = 0, off_gu2 = 0, off_gcnt = 0, off_gval = 0, off_gncnt = 0, off_gnlist = 0;
   size_t off_alt = 0;
   bool split_tree = false;  // tree stage as sort + hierarchy kernels (sorted-edge transport)
   int sort_threads = 0, hier_threads = 0;
@@ -317,9 +321,17 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   // interval-key Prim's rounds cost more than the exact matrix saves).
   P.gram_graph = P.matrix && rs == 4 && d <= 256 && ms <= 64 && (flags & HDB_FLAG_GRAM_GRAPH) &&
                  !(flags & (HDB_FLAG_PLAIN | HDB_FLAG_MATRIX_INPUT | HDB_FLAG_ONE_STEP)) && want_vpt != 8;
+  // The MFMA-bounded matrix (hdb_gramexact.h): float32 matrix path, core
+  // distances for min_samples <= 16 (or none), up to 256 dimensions.
+  P.gram_exact = P.matrix && rs == 4 && !P.gram_graph && d <= 256 && ms <= 16 && want_vpt != 8 &&
+                 !(flags & (HDB_FLAG_PLAIN | HDB_FLAG_MATRIX_INPUT | HDB_FLAG_ONE_STEP | HDB_FLAG_NO_GRAM_EXACT));
   if (P.gram_graph) {
     P.gram_dk = d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
     if (P.vpt > 4) return fail("window too large for the vertex layout");
+  } else if (P.gram_exact) {
+    P.gx_dk = d <= 32 ? 32 : d <= 64 ? 64 : d <= 128 ? 128 : 256;
+    P.gx_cols = (int)n;
+    P.gx_rank = std::max<int>(ms, 8);
   } else {
     set_gram(P, n, d, ms, lds_limit);
   }
@@ -360,6 +372,13 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.off_pb = o; o = align256(o + pieces);
   P.off_nb = o; o = align256(o + (pieces_needed ? 3 * bn * 4 : 0));
   P.off_alt = o; o = align256(o + (P.gram_graph ? bn * 4 : 0));
+  P.off_gop = o; o = align256(o + (P.gram_exact ? gx_operand_entries(P.gx_dk, batch, n) * 16 : 0));
+  P.off_gnorm = o; o = align256(o + (P.gram_exact ? bn * 4 : 0));
+  P.off_gu2 = o; o = align256(o + (P.gram_exact ? bn * 4 : 0));
+  P.off_gcnt = o; o = align256(o + (P.gram_exact ? bn * 4 : 0));
+  P.off_gval = o; o = align256(o + (P.gram_exact ? bn * kGxCoreCap * 4 : 0));
+  P.off_gncnt = o; o = align256(o + (P.gram_exact ? (size_t)batch * 4 : 0));
+  P.off_gnlist = o; o = align256(o + (P.gram_exact ? bn * kGxNearPerPoint * 4 : 0));
   // Split tree stage when windows outnumber the compute units at least two
   // to one: its narrow hierarchy workgroups then share CUs (B=1024 windows of
   // 1024 points: tree 0.73 -> 0.27 ms; at B=256 the fused kernel is 10 %
@@ -713,6 +732,112 @@ int launch_tree(hipStream_t s, const Plan& P, int64_t batch, const TreeArgs& a) 
   return check_launch("tree_kernel");
 }
 
+// The MFMA-bounded matrix, core distances from its lists, and Prim over it
+// (hdb_gramexact.h).  Dense batches only.
+template <int DK>
+int launch_gx_matrix(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d,
+                     unsigned char* ws, int32_t* status) {
+  const int nblk = (int)((n + 15) / 16);
+  const gram_s16x8* op = reinterpret_cast<const gram_s16x8*>(ws + P.off_gop);
+  const float* norms = reinterpret_cast<const float*>(ws + P.off_gnorm);
+  float* u2 = reinterpret_cast<float*>(ws + P.off_gu2);
+  constexpr int RB = DK >= 128 ? 2 : 4;
+  const float c2 = gram_c2(DK);
+  gx_bound_kernel<DK, RB><<<xcd_grid(batch, (int)((n + 64 * RB - 1) / (64 * RB))), 256, 0, s>>>(
+      op, norms, batch, (int)n, nblk, P.gx_cols, P.gx_rank, c2 + kGxPassA, u2);
+  if (int r = check_launch("gx_bound_kernel")) return r;
+  stage_mark(s, "gx_bound_kernel");
+  const unsigned nt = (unsigned)((n + kGxTile - 1) / kGxTile);
+  float* M = reinterpret_cast<float*>(ws + P.off_matrix);
+  int32_t* lcnt = reinterpret_cast<int32_t*>(ws + P.off_gcnt);
+  float* lval = reinterpret_cast<float*>(ws + P.off_gval);
+  int32_t* ncnt = reinterpret_cast<int32_t*>(ws + P.off_gncnt);
+  uint32_t* nlist = reinterpret_cast<uint32_t*>(ws + P.off_gnlist);
+  const int cap = kGxNearPerPoint * (int)n;
+  // Persistent: four workgroups per CU (a multiple of 8 keeps each on its XCD).
+  const unsigned tiles = xcd_grid(batch, (int)(nt * (nt + 1) / 2));
+  gx_matrix_kernel<DK><<<std::min<unsigned>(tiles, 4u * (unsigned)((device_cus() + 7) / 8 * 8)), 256, 0, s>>>(
+      x, batch, (int)n, (int)d, nblk, op, norms, u2, M, lcnt, lval, ncnt, nlist, cap, status);
+  if (int r = check_launch("gx_matrix_kernel")) return r;
+  stage_mark(s, "gx_matrix_kernel");
+  gx_near_kernel<<<xcd_grid(batch, (cap + 255) / 256), 256, 0, s>>>(x, batch, (int)n, (int)d, cap, ncnt, nlist, u2,
+                                                                   M, lcnt, lval, status);
+  return check_launch("gx_near_kernel");
+}
+
+template <int DK>
+int launch_gx_prep(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d,
+                   unsigned char* ws, int32_t* status) {
+  const int nblk = (int)((n + 15) / 16);
+  gx_prep_kernel<DK><<<dim3((unsigned)nblk, (unsigned)batch), 256, 0, s>>>(
+      x, (int)n, (int)d, nblk, reinterpret_cast<gram_s16x8*>(ws + P.off_gop),
+      reinterpret_cast<float*>(ws + P.off_gnorm), reinterpret_cast<int32_t*>(ws + P.off_gcnt),
+      reinterpret_cast<int32_t*>(ws + P.off_gncnt), status);
+  return check_launch("gx_prep_kernel");
+}
+
+template <int VPT>
+int launch_prim_gx(hipStream_t s, const Plan& P, int64_t batch, const GxPrimArgs& a) {
+  auto fn = prim_gx_kernel<VPT>;
+  if (int r = allow_lds((const void*)fn, P.prim_lds)) return r;
+  fn<<<(unsigned)batch, P.threads, P.prim_lds, s>>>(a);
+  return check_launch("prim_gx_kernel");
+}
+
+int run_gram_exact(hipStream_t s, const Plan& P, const float* x, const Launch& L, int64_t d, int ms, bool unit,
+                   unsigned char* ws, int32_t* status) {
+  const int64_t batch = L.count, n = L.nmax, points = L.points;
+  if (points != batch * n) return fail("internal: ragged batch on the MFMA-bounded matrix path");
+  int r = 0;
+  switch (P.gx_dk) {
+    case 32: r = launch_gx_prep<32>(s, P, x, batch, n, d, ws, status); break;
+    case 64: r = launch_gx_prep<64>(s, P, x, batch, n, d, ws, status); break;
+    case 128: r = launch_gx_prep<128>(s, P, x, batch, n, d, ws, status); break;
+    default: r = launch_gx_prep<256>(s, P, x, batch, n, d, ws, status); break;
+  }
+  if (r) return r;
+  stage_mark(s, "gx_prep_kernel");
+  switch (P.gx_dk) {
+    case 32: r = launch_gx_matrix<32>(s, P, x, batch, n, d, ws, status); break;
+    case 64: r = launch_gx_matrix<64>(s, P, x, batch, n, d, ws, status); break;
+    case 128: r = launch_gx_matrix<128>(s, P, x, batch, n, d, ws, status); break;
+    default: r = launch_gx_matrix<256>(s, P, x, batch, n, d, ws, status); break;
+  }
+  if (r) return r;
+  stage_mark(s, "gx_near_kernel");
+  const float* M = reinterpret_cast<const float*>(ws + P.off_matrix);
+  float* core = reinterpret_cast<float*>(ws + P.off_core);
+  if (!unit) {
+    gx_core_kernel<16><<<(unsigned)((points + 3) / 4), 256, 0, s>>>(
+        M, points, (int)n, ms, reinterpret_cast<const int32_t*>(ws + P.off_gcnt),
+        reinterpret_cast<const float*>(ws + P.off_gval), core);
+    if (int e = check_launch("gx_core_kernel")) return e;
+    stage_mark(s, "gx_core_kernel");
+  }
+  GxPrimArgs a;
+  a.m = M;
+  a.x = x;
+  a.core = unit ? nullptr : core;
+  a.norms = reinterpret_cast<const float*>(ws + P.off_gnorm);
+  a.n = (int)n;
+  a.d = (int)d;
+  a.unit_core = unit ? 1 : 0;
+  a.c2 = gram_c2(P.gx_dk);
+  a.edge_u = reinterpret_cast<int32_t*>(ws + P.off_eu);
+  a.edge_v = reinterpret_cast<int32_t*>(ws + P.off_ev);
+  a.edge_w = reinterpret_cast<float*>(ws + P.off_ew);
+  a.wflag = reinterpret_cast<int32_t*>(ws + P.off_flag);
+  a.status = status;
+  switch (P.vpt) {
+    case 1: r = launch_prim_gx<1>(s, P, batch, a); break;
+    case 2: r = launch_prim_gx<2>(s, P, batch, a); break;
+    default: r = launch_prim_gx<4>(s, P, batch, a); break;
+  }
+  if (r) return r;
+  stage_mark(s, "prim_kernel");
+  return 0;
+}
+
 template <typename Real>
 int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_dist, const Real* given_core,
                 const Launch& L, int64_t d, long long mcs, int ms, bool unit, double eps, int sort_depth,
@@ -732,6 +857,10 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
     if (P.gram_graph && !given_dist) {
       if (L.win.off) return fail("internal: ragged batch on the matrix path");
       if (int r = run_gram_graph(s, P, x, L, d, ms, unit, ws, status)) return r;
+      prim_done = true;
+    } else if (P.gram_exact && !given_dist) {
+      if (L.win.off) return fail("internal: ragged batch on the MFMA-bounded matrix path");
+      if (int r = run_gram_exact(s, P, x, L, d, ms, unit, ws, status)) return r;
       prim_done = true;
     }
   }
@@ -887,9 +1016,9 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
     tuned = g_tune_set;
     for (int k = 0; k < kBuckets; ++k) tune[k] = g_tune[k];
   }
-  // The Gram graph stage lays matrices out densely; ragged buckets on the
-  // matrix path use the exact matrix.
-  const int base_flags = flags & ~HDB_FLAG_GRAM_GRAPH;
+  // The Gram graph stage and the MFMA-bounded matrix lay matrices out
+  // densely; ragged buckets on the matrix path use the exact matrix.
+  const int base_flags = (flags & ~HDB_FLAG_GRAM_GRAPH) | HDB_FLAG_NO_GRAM_EXACT;
   for (int k = 0; k < kBuckets; ++k) {
     if (!R.used[k]) continue;
     const int fk = tuned ? (base_flags & ~kTuneMask) | (tune[k] & kTuneMask) : base_flags;

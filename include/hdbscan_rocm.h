@@ -57,6 +57,7 @@ extern "C" {
 #define HDB_FLAG_SPLIT_TREE 1048576 /* tree stage as a sort kernel + a narrow hierarchy kernel (sorted-edge transport) */
 #define HDB_FLAG_FUSED_TREE 2097152 /* one tree kernel per window even when windows outnumber the CUs */
 #define HDB_FLAG_SERIAL_LINKAGE 4194304 /* linkage by one wave in sort order (the parallel run form is default) */
+#define HDB_FLAG_NO_GRAM_EXACT 8388608 /* float32 matrix path: evaluate every pair exactly (no MFMA-bounded near/far matrix) */
 /* Bits 8..11: vertices per Prim thread (1, 2 or 4; 8 on the speculative
  * matrix path only); 0 = automatic.
  * Bits 12..13: 16-row sets per wave of the Gram core kernel (1 -> 1, 2 -> 2,
