@@ -386,6 +386,51 @@ def distance_error(xd, x, precision, torch, windows=2):
             "unit_roundoff_fp32": 2.0 ** -24}
 
 
+def bounded_error(xd, x, ms, torch, windows=2):
+    """The MFMA-bounded matrix against a float64 evaluation of the same
+    points: the share of pairs stored exactly, the exact entries' largest
+    relative error, and how far below the float64 distance the certified
+    lower bounds sit (relative gap), with the count of bounds above it."""
+    from hdbscan_amd import _native
+
+    sub = xd[:windows].contiguous()
+    b, n, d = (int(s) for s in sub.shape)
+    nbytes = _native.lib().hdb_workspace_bytes(b, n, d, _native.F32, int(ms), _native.FLAG_MATRIX)
+    ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device="cuda")
+    out = torch.empty((b, n, n), dtype=torch.float32, device="cuda")
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _native.check(_native.lib().hdb_bounded_distances(None, ptr(sub), b, n, d, _native.F32, int(ms), ptr(out),
+                                                      ptr(ws), int(nbytes), ptr(status)))
+    got = to_host(out, torch)
+    exact_pairs = total = over = 0
+    worst_exact = 0.0
+    gaps = []
+    for w in range(b):
+        p = x[w].astype(np.float64)
+        sq = (p * p).sum(1)
+        ref = np.sqrt(np.maximum(sq[:, None] + sq[None, :] - 2.0 * p @ p.T, 0.0))
+        close = ref < 1e-3 * np.sqrt(sq.max() + 1e-300)
+        if close.any():
+            i, j = np.nonzero(close)
+            ref[i, j] = np.sqrt(((p[i] - p[j]) ** 2).sum(1))
+        off = ~np.eye(n, dtype=bool)
+        m = got[w].astype(np.float64)
+        far = np.signbit(got[w]) & off
+        near = ~np.signbit(got[w]) & off & (ref > 0)
+        exact_pairs += int(near.sum())
+        total += int(off.sum())
+        if near.any():
+            worst_exact = max(worst_exact, float((np.abs(m[near] - ref[near]) / ref[near]).max()))
+        if far.any():
+            lb = -m[far]
+            over += int((lb > ref[far] * (1 + 2.0 ** -22)).sum())
+            gaps.append((ref[far] - lb) / np.maximum(ref[far], 1e-300))
+    g = np.concatenate(gaps) if gaps else np.zeros(1)
+    return {"windows": b, "exact_pair_fraction": exact_pairs / max(total, 1), "exact_max_rel": worst_exact,
+            "lower_bound_rel_gap_max": float(g.max()), "lower_bound_rel_gap_median": float(np.median(g)),
+            "lower_bounds_above_fp64": over}
+
+
 # --------------------------------------------------------------------------
 # Ragged batches
 
@@ -614,7 +659,7 @@ def main(argv=None):
     log(f"VALU issue ceiling (sub/mul/add mix): {tops:.2f} Top/s ({probe})")
     dominant = max(stages, key=stages.get)
     roofline = roofline_for(dominant, stages[dominant], batch, n, d, precision, valu_tops=tops,
-                            matrix="distance_kernel" in stages)
+                            matrix="distance_kernel" in stages or "gx_matrix_kernel" in stages)
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
@@ -637,6 +682,8 @@ def main(argv=None):
     log(f"public API ({len(chunks)} x B={per_call}): median {api_rep['median_batch_ms']:.3f} ms")
 
     dist_err = distance_error(xd, x, precision, torch) if d >= 64 else None
+    if dist_err is not None and "gx_matrix_kernel" in stages:
+        dist_err["bounded_matrix"] = bounded_error(xd, x, ms, torch)
 
     cpu = None
     oracle_cpu = None
@@ -680,8 +727,9 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": args.workload, "batch": batch, "n": n, "d": d,
                    "min_cluster_size": mcs, "min_samples": ms, "cluster_selection_epsilon": eps,
-                   "path": "MFMA Gram filter" if any(k.startswith("gram") or k.startswith("core_gram")
-                                                     for k in stages) else "plain exact-distance",
+                   "path": ("MFMA-bounded matrix" if "gx_matrix_kernel" in stages else
+                            "MFMA Gram filter" if any(k.startswith("gram") or k.startswith("core_gram")
+                                                      for k in stages) else "plain exact-distance"),
                    "flags": args.flags},
         "label_exact_windows": exact,
         "timing": {"resident": main_run["resident"], "host_to_host": host_to_host,

@@ -24,7 +24,7 @@ FLAG_NO_GRAM_EXACT = 8388608
 EXPORTED = ("hdb_version", "hdb_last_error", "hdb_profile", "hdb_stage_count", "hdb_stage_name",
             "hdb_stage_ms", "hdb_workspace_bytes", "hdb_cluster", "hdb_cluster_export", "hdb_distances",
             "hdb_core_distances", "hdb_cluster_matrix", "hdb_ragged_workspace_bytes", "hdb_cluster_ragged",
-            "hdb_cluster_noalloc", "hdb_probe_valu", "hdb_ragged_set_tuning")
+            "hdb_cluster_noalloc", "hdb_probe_valu", "hdb_ragged_set_tuning", "hdb_bounded_distances")
 NOALLOC_BYTES = 64 * 1024 * 1024
 
 _lib = None
@@ -72,6 +72,8 @@ def lib():
                                               _P, _P, ctypes.c_size_t, _P, _I, _P, _P]
         handle.hdb_distances.restype = _I
         handle.hdb_distances.argtypes = [_P, _P, _I64, _I64, _I64, _I, _P, _P]
+        handle.hdb_bounded_distances.restype = _I
+        handle.hdb_bounded_distances.argtypes = [_P, _P, _I64, _I64, _I64, _I, _I32, _P, _P, ctypes.c_size_t, _P]
         handle.hdb_core_distances.restype = _I
         handle.hdb_core_distances.argtypes = [_P, _P, _I64, _I64, _I64, _I, _I32, _P, _P,
                                               ctypes.c_size_t, _P, _I]

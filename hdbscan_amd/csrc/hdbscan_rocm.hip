@@ -736,7 +736,7 @@ int launch_tree(hipStream_t s, const Plan& P, int64_t batch, const TreeArgs& a) 
 // (hdb_gramexact.h).  Dense batches only.
 template <int DK>
 int launch_gx_matrix(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d,
-                     unsigned char* ws, int32_t* status) {
+                     unsigned char* ws, float* M, int32_t* status) {
   const int nblk = (int)((n + 15) / 16);
   const gram_s16x8* op = reinterpret_cast<const gram_s16x8*>(ws + P.off_gop);
   const float* norms = reinterpret_cast<const float*>(ws + P.off_gnorm);
@@ -748,7 +748,6 @@ int launch_gx_matrix(hipStream_t s, const Plan& P, const float* x, int64_t batch
   if (int r = check_launch("gx_bound_kernel")) return r;
   stage_mark(s, "gx_bound_kernel");
   const unsigned nt = (unsigned)((n + kGxTile - 1) / kGxTile);
-  float* M = reinterpret_cast<float*>(ws + P.off_matrix);
   int32_t* lcnt = reinterpret_cast<int32_t*>(ws + P.off_gcnt);
   float* lval = reinterpret_cast<float*>(ws + P.off_gval);
   int32_t* ncnt = reinterpret_cast<int32_t*>(ws + P.off_gncnt);
@@ -784,10 +783,9 @@ int launch_prim_gx(hipStream_t s, const Plan& P, int64_t batch, const GxPrimArgs
   return check_launch("prim_gx_kernel");
 }
 
-int run_gram_exact(hipStream_t s, const Plan& P, const float* x, const Launch& L, int64_t d, int ms, bool unit,
-                   unsigned char* ws, int32_t* status) {
-  const int64_t batch = L.count, n = L.nmax, points = L.points;
-  if (points != batch * n) return fail("internal: ragged batch on the MFMA-bounded matrix path");
+// Prep, pass A, the bounded matrix M and the exact near pairs.
+int run_gram_matrix_only(hipStream_t s, const Plan& P, const float* x, int64_t batch, int64_t n, int64_t d,
+                         unsigned char* ws, float* M, int32_t* status) {
   int r = 0;
   switch (P.gx_dk) {
     case 32: r = launch_gx_prep<32>(s, P, x, batch, n, d, ws, status); break;
@@ -798,13 +796,22 @@ int run_gram_exact(hipStream_t s, const Plan& P, const float* x, const Launch& L
   if (r) return r;
   stage_mark(s, "gx_prep_kernel");
   switch (P.gx_dk) {
-    case 32: r = launch_gx_matrix<32>(s, P, x, batch, n, d, ws, status); break;
-    case 64: r = launch_gx_matrix<64>(s, P, x, batch, n, d, ws, status); break;
-    case 128: r = launch_gx_matrix<128>(s, P, x, batch, n, d, ws, status); break;
-    default: r = launch_gx_matrix<256>(s, P, x, batch, n, d, ws, status); break;
+    case 32: r = launch_gx_matrix<32>(s, P, x, batch, n, d, ws, M, status); break;
+    case 64: r = launch_gx_matrix<64>(s, P, x, batch, n, d, ws, M, status); break;
+    case 128: r = launch_gx_matrix<128>(s, P, x, batch, n, d, ws, M, status); break;
+    default: r = launch_gx_matrix<256>(s, P, x, batch, n, d, ws, M, status); break;
   }
   if (r) return r;
   stage_mark(s, "gx_near_kernel");
+  return 0;
+}
+
+int run_gram_exact(hipStream_t s, const Plan& P, const float* x, const Launch& L, int64_t d, int ms, bool unit,
+                   unsigned char* ws, int32_t* status) {
+  const int64_t batch = L.count, n = L.nmax, points = L.points;
+  if (points != batch * n) return fail("internal: ragged batch on the MFMA-bounded matrix path");
+  int r = run_gram_matrix_only(s, P, x, batch, n, d, ws, reinterpret_cast<float*>(ws + P.off_matrix), status);
+  if (r) return r;
   const float* M = reinterpret_cast<const float*>(ws + P.off_matrix);
   float* core = reinterpret_cast<float*>(ws + P.off_core);
   if (!unit) {
@@ -1451,6 +1458,22 @@ int hdb_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t
                                    static_cast<float*>(out), status);
   return launch_distances<double>(s, static_cast<const double*>(x), dense_launch(batch, n), d,
                                   static_cast<double*>(out), status);
+}
+
+int hdb_bounded_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,
+                          int32_t min_samples, void* out, void* workspace, size_t workspace_bytes,
+                          int32_t* status) {
+  g_error.clear();
+  if (!x || !out || !status) return fail("null pointer argument");
+  if (dtype != HDB_F32) return fail("the bounded matrix is a float32 form");
+  Plan P;
+  if (make_plan(batch, n, d, dtype, min_samples, HDB_FLAG_MATRIX, &P)) return -1;
+  if (!P.gram_exact) return fail("these arguments do not take the bounded matrix (min_samples <= 16, d <= 256)");
+  if (workspace_bytes < P.total || (P.total && !workspace)) return fail("workspace too small");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
+  return run_gram_matrix_only(s, P, static_cast<const float*>(x), batch, n, d, ws, static_cast<float*>(out),
+                              status);
 }
 
 int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, int64_t d, int dtype,

@@ -142,6 +142,20 @@ int hdb_cluster_export(void *stream, const void *x, int64_t batch, int64_t n,
 int hdb_distances(void *stream, const void *x, int64_t batch, int64_t n,
                   int64_t d, int dtype, void *out, int32_t *status);
 
+/* The MFMA-bounded matrix of the float32 matrix path, out[batch][n][n]:
+ * for each pair either its exact distance (bitwise hdb_distances' value,
+ * sign bit clear) or minus a certified lower bound of it (sign bit set),
+ * as hdb_cluster's float32 matrix path stores it for min_samples <= 16
+ * (which pairs are exact depends on min_samples).  Workspace:
+ * hdb_workspace_bytes(batch, n, d, HDB_F32, min_samples, HDB_FLAG_MATRIX).
+ * A diagnostic with no counterpart in the reference (whose matrix is always
+ * exact, hdbscan_gpu/_kernels.py:45-88); it exposes the bounds for tests and
+ * for measuring their error against a float64 evaluation. */
+int hdb_bounded_distances(void *stream, const void *x, int64_t batch, int64_t n,
+                          int64_t d, int dtype, int32_t min_samples, void *out,
+                          void *workspace, size_t workspace_bytes,
+                          int32_t *status);
+
 /* Core distances core[batch][n]: the (min_samples-1)-th smallest distance of
  * each row, self included.  Workspace: hdb_workspace_bytes(batch, n, d,
  * dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_FILTER) bytes

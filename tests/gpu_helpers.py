@@ -66,6 +66,26 @@ def distances(x, precision):
     return host(out), int(host(status)[0])
 
 
+def bounded_distances(x, min_samples):
+    """hdb_bounded_distances (float32): [B, N, N] exact distances or minus
+    certified lower bounds, and the status."""
+    t = torch()
+    x = np.asarray(x, np.float32)
+    if x.ndim == 2:
+        x = x[None]
+    b, n, d = x.shape
+    lib = _native.lib()
+    ws_bytes = lib.hdb_workspace_bytes(b, n, d, _native.F32, int(min_samples), _native.FLAG_MATRIX)
+    _native.check(0 if ws_bytes else -1)
+    xd = dev(x)
+    ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
+    out = t.empty((b, n, n), dtype=t.float32, device="cuda")
+    status = t.zeros(1, dtype=t.int32, device="cuda")
+    _native.check(lib.hdb_bounded_distances(stream(), ptr(xd), b, n, d, _native.F32, int(min_samples), ptr(out),
+                                            ptr(ws), ws_bytes, ptr(status)))
+    return host(out), int(host(status)[0])
+
+
 def core(x, min_samples, precision, flags=0):
     t = torch()
     x = np.asarray(x, real(precision))

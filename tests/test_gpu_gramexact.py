@@ -130,6 +130,26 @@ def test_bounded_matrix_batch_of_windows():
         np.testing.assert_array_equal(labels[b], expected, err_msg=f"window {b}")
 
 
+@pytest.mark.parametrize("name", [k for k in WINDOWS if len(WINDOWS[k]) >= 8])
+@pytest.mark.parametrize("ms", [3, 8, 16])
+def test_bounded_matrix_entries_are_exact_or_certified(name, ms):
+    """Every entry of the bounded matrix is either the oracle's exact
+    distance, bit for bit (sign bit clear), or minus a lower bound of it
+    (sign bit set); the diagonal is +0 and both halves agree.  Cancellation
+    (offset windows), norms near and past the bounded range, ties and
+    duplicates exercise the MFMA certificate."""
+    x = WINDOWS[name]
+    got, status = G.bounded_distances(x, ms)
+    assert status == 0
+    m = got[0]
+    exact = oracle.distances(x, "float32")
+    far = np.signbit(m)
+    np.testing.assert_array_equal(m[~far].view(np.uint32), exact[~far].view(np.uint32))
+    assert (-m[far] <= exact[far]).all(), float((-m[far] - exact[far]).max())
+    np.testing.assert_array_equal(m.view(np.uint32), m.T.view(np.uint32))
+    assert (np.diag(m).view(np.uint32) == 0).all()
+
+
 @pytest.mark.parametrize("value,bit", [(np.nan, 2), (np.inf, 2), (-np.inf, 2), (1e20, 1)])
 def test_bounded_matrix_status(value, bit):
     """Non-finite input (bit 2, from the prep kernel) and a distance that
