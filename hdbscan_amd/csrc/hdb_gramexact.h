@@ -437,6 +437,45 @@ void gx_matrix_kernel(const float* x, int64_t batch, int n, int d, int nblk, con
 // and in window order there, so that a window's points stay in that XCD's
 // L2 while its pairs are evaluated; chunks past a window's count exit.
 // Grid: xcd_grid(batch, ceil(cap / 256)).
+// Exact coordinate-order squared distance of rows xi and xj for d <= DK:
+// the loads of 64 coordinates at a time are all issued before the first add
+// (the row pairs of one wave are scattered, so each phase costs one memory
+// round trip); the adds run in coordinate order, t = t + dl * dl.
+template <int DK>
+__device__ __forceinline__ float gx_exact_sq(const float* xi, const float* xj, int d) {
+  constexpr int PH = DK < 64 ? DK : 64;
+  const bool vec = ((d & 3) == 0) && ((((uintptr_t)xi) | ((uintptr_t)xj)) & 15) == 0;
+  float t = 0.f;
+#pragma unroll
+  for (int k0 = 0; k0 < DK; k0 += PH) {
+    if (k0 >= d) break;  // d is uniform
+    float a[PH], c[PH];
+    if (vec && k0 + PH <= d) {
+#pragma unroll
+      for (int q = 0; q < PH / 4; ++q) {
+        const float4 va = reinterpret_cast<const float4*>(xi + k0)[q];
+        const float4 vc = reinterpret_cast<const float4*>(xj + k0)[q];
+        a[4 * q] = va.x; a[4 * q + 1] = va.y; a[4 * q + 2] = va.z; a[4 * q + 3] = va.w;
+        c[4 * q] = vc.x; c[4 * q + 1] = vc.y; c[4 * q + 2] = vc.z; c[4 * q + 3] = vc.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PH; ++k) {
+        a[k] = k0 + k < d ? xi[k0 + k] : 0.f;
+        c[k] = k0 + k < d ? xj[k0 + k] : 0.f;
+      }
+    }
+    // Padding adds (0 - 0)^2 = +0, which leaves the sum unchanged.
+#pragma unroll
+    for (int k = 0; k < PH; ++k) {
+      const float dl = a[k] - c[k];
+      t = t + dl * dl;
+    }
+  }
+  return t;
+}
+
+template <int DK>
 __global__ __launch_bounds__(256) void gx_near_kernel(const float* x, int64_t batch, int n, int d, int cap,
                                                       const int32_t* ncnt, const uint32_t* nlist, const float* U2,
                                                       float* M, int32_t* lcnt, float* lval, int32_t* status) {
@@ -450,7 +489,7 @@ __global__ __launch_bounds__(256) void gx_near_kernel(const float* x, int64_t ba
     const uint32_t pe = nlist[(size_t)b * cap + e];
     const int i = (int)(pe >> 12), j = (int)(pe & 4095u);
     const int64_t base = (int64_t)b * n;
-    const float sq = exact_sq(x + (size_t)(base + i) * d, x + (size_t)(base + j) * d, d);
+    const float sq = gx_exact_sq<DK>(x + (size_t)(base + i) * d, x + (size_t)(base + j) * d, d);
     bad = !isfinite(sq);
     const float dv = sqrtf(sq);
     float* mb = M + (size_t)base * n;
@@ -507,11 +546,11 @@ struct GxPrimArgs {
   int32_t* status;
 };
 
-// Key (weight, vertex) with the "floor not above the key" flag in the lowest
+// Key (weight bits, vertex) with the lower-bound flag (0 or 1) in the lowest
 // bit of the vertex word: keys order by weight, then vertex.
-__device__ __forceinline__ Cand<float> gx_key(float w, int j, bool flag) {
+__device__ __forceinline__ Cand<float> gx_key(uint32_t wb, int j, uint32_t flag) {
   Cand<float> c;
-  c.k = ((uint64_t)__float_as_uint(w) << 32) | ((uint32_t)j << 1) | (flag ? 1u : 0u);
+  c.k = ((uint64_t)wb << 32) | ((uint32_t)j << 1) | flag;
   return c;
 }
 
@@ -523,20 +562,6 @@ __device__ __forceinline__ uint64_t gx_block_umin64(uint64_t v, uint64_t* sh, in
   uint64_t r = ~0ull;
   for (int w = 0; w < W; ++w) r = min(r, sh[w]);
   return r;
-}
-
-// Relaxation of one row entry v (exact distance, or minus a lower bound) of
-// the row of visited vertex c for an unvisited vertex with key b, source s
-// and flag bit `bit` of fl; m = max of the two core distances (0 with unit
-// cores).  A weight strictly below the key replaces it; the flag records
-// whether it is a lower bound.  (Non-finite distances are reported by the
-// matrix kernel.)
-__device__ __forceinline__ void gx_relax(float v, float m, int c, float& b, int& s, uint32_t& fl, uint32_t bit) {
-  const float w = fmaxf(m, fabsf(v));
-  const bool lt = w < b;
-  b = lt ? w : b;
-  s = lt ? c : s;
-  fl = lt ? (__float_as_int(v) < 0 ? fl | bit : fl & ~bit) : fl;
 }
 
 template <int VPT>
@@ -559,16 +584,23 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
   if (tid == 0) sh_err = 0;
   if (!unit)
     for (int i = tid; i < n; i += T) cs[i] = a.core[base + i];
-  float best[VPT], cr[VPT];
-  int src[VPT], vst[VPT];
-  uint32_t vis = 0, flg = 0;
+  // Keys, cores and row entries are handled as bit patterns: every weight is
+  // a non-negative float (or +inf), whose order is that of its bits, so max
+  // and < are single unsigned instructions.  A slot's flag is 0 or 1.
+  uint32_t bb[VPT], crb[VPT], fg[VPT];
+  float cr[VPT];
+  int src[VPT], vst[VPT], jc[VPT];
+  uint32_t vis = 0;
 #pragma unroll
   for (int s = 0; s < VPT; ++s) {
     const int j = tid + s * T;
-    best[s] = INFINITY;
+    bb[s] = 0x7F800000u;  // +inf
+    fg[s] = 0;
     src[s] = 1;
     vst[s] = 0;
+    jc[s] = min(j, n - 1);
     cr[s] = j < n && !unit ? a.core[base + j] : 0.f;
+    crb[s] = __float_as_uint(cr[s]);
     if (j >= n) vis |= 1u << s;
   }
   if (tid == 0) vis |= 1u;  // vertex 0 starts the tree (visit step 0)
@@ -660,60 +692,58 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
 #pragma unroll
     for (int s = 0; s < VPT; ++s)
       if (tid + s * T == r) {
-        best[s] = __uint_as_float((uint32_t)(key >> 32));
+        bb[s] = (uint32_t)(key >> 32);
         src[s] = (int)(key & 0xFFFu);
-        flg &= ~(1u << s);
+        fg[s] = 0;
       }
   };
 
   for (int step = 0; step < n - 1;) {
+    cur = __builtin_amdgcn_readfirstlane(cur);  // uniform: row addresses in scalar registers
+    pr = __builtin_amdgcn_readfirstlane(pr);
     const bool spec = !rowless && pr >= 0 && n - 1 - step >= 2;
-    const float ccur = rowless || unit ? 0.f : cs[cur];
-    const float cpr = spec && !unit ? cs[pr] : 0.f;
-    // Read conditions of every slot first, then all row loads together, so
-    // that no wait separates the loads of one slot from those of another.
-    float d1[VPT], d2[VPT];
-    bool l1[VPT], l2[VPT];
+    const uint32_t ccur = rowless || unit ? 0u : __float_as_uint(cs[cur]);
+    const uint32_t cpr = spec && !unit ? __float_as_uint(cs[pr]) : 0u;
+    // Both rows are read whole (clamped index past the window), all loads
+    // issued together; entries are used only where they can lower a key.
+    uint32_t d1[VPT], d2[VPT];
+    const float* row1 = mg + (size_t)cur * n;
+    const float* row2 = mg + (size_t)(spec ? pr : cur) * n;
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
-      const int j = tid + s * T;
-      const bool un = !((vis >> s) & 1u);
-      l1[s] = !rowless && un && (unit || fmaxf(ccur, cr[s]) < best[s]);
-      l2[s] = spec && un && j != pr && (unit || fmaxf(cpr, cr[s]) < best[s]);
+      d1[s] = rowless ? 0u : __float_as_uint(row1[jc[s]]);
+      d2[s] = spec ? __float_as_uint(row2[jc[s]]) : 0u;
     }
-    const float* row1 = mg + (size_t)(rowless ? 0 : cur) * n;
-    const float* row2 = mg + (size_t)(spec ? pr : 0) * n;
-#pragma unroll
-    for (int s = 0; s < VPT; ++s) {
-      d1[s] = l1[s] ? row1[tid + s * T] : 0.f;
-      d2[s] = l2[s] ? row2[tid + s * T] : 0.f;
-    }
-    float b1[VPT], b2[VPT];
+    uint32_t b1[VPT], b2[VPT], f1[VPT], f2[VPT];
     int s1[VPT], s2[VPT];
-    uint32_t fm1 = flg, fm2 = flg;
     C k1a = none, k1b = none, k2a = none, k2b = none;
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
       const int j = tid + s * T;
-      b1[s] = best[s];
-      s1[s] = src[s];
-      b2[s] = best[s];
-      s2[s] = src[s];
-      if (!((vis >> s) & 1u)) {
-        if (l1[s]) gx_relax(d1[s], unit ? 0.f : fmaxf(ccur, cr[s]), cur, b1[s], s1[s], fm1, 1u << s);
-        const C o1 = gx_key(b1[s], j, (fm1 >> s) & 1u);
-        if (spec)
-          k1a = cand_pick(o1.less(k1a), o1, k1a);
-        else
-          cand_top2(k1a, k1b, o1);
-        if (spec && j != pr) {
-          b2[s] = b1[s];
-          s2[s] = s1[s];
-          fm2 = (fm2 & ~(1u << s)) | (fm1 & (1u << s));
-          if (unit || fmaxf(cpr, cr[s]) < b1[s])
-            gx_relax(d2[s], unit ? 0.f : fmaxf(cpr, cr[s]), pr, b2[s], s2[s], fm2, 1u << s);
-          cand_top2(k2a, k2b, gx_key(b2[s], j, (fm2 >> s) & 1u));
-        }
+      const bool un = !((vis >> s) & 1u);
+      // Relaxation by c's row: w = max(core_c, core_j, |entry|); a weight
+      // strictly below the key replaces it, the entry's sign bit its flag.
+      const uint32_t w1 = max(max(ccur, crb[s]), d1[s] & 0x7FFFFFFFu);
+      const bool lt1 = !rowless && un && w1 < bb[s];
+      b1[s] = lt1 ? w1 : bb[s];
+      s1[s] = lt1 ? cur : src[s];
+      f1[s] = lt1 ? d1[s] >> 31 : fg[s];
+      const C o1 = un ? gx_key(b1[s], j, f1[s]) : C::none();
+      if (spec) {
+        k1a = cand_pick(o1.less(k1a), o1, k1a);
+        // ... and then by p's row, for p's runner-up step.
+        const bool e2 = un && j != pr;
+        const uint32_t w2 = max(max(cpr, crb[s]), d2[s] & 0x7FFFFFFFu);
+        const bool lt2 = e2 && w2 < b1[s];
+        b2[s] = lt2 ? w2 : b1[s];
+        s2[s] = lt2 ? pr : s1[s];
+        f2[s] = lt2 ? d2[s] >> 31 : f1[s];
+        cand_top2(k2a, k2b, e2 ? gx_key(b2[s], j, f2[s]) : C::none());
+      } else {
+        b2[s] = b1[s];
+        s2[s] = s1[s];
+        f2[s] = f1[s];
+        cand_top2(k1a, k1b, o1);
       }
     }
     const C pa = cand_pick(spec, k2a, k1a), pb = cand_pick(spec, k2b, k1b);
@@ -739,8 +769,8 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
     const C first = cand_pick(spec, gc, g1);
     const int v1 = first.index() >> 1;
     const bool fl1 = first.index() & 1;
-    // A winner whose floor is not above its key is not added: this step's
-    // keys are kept (c's row; p's row is dropped) and it is resolved below.
+    // A winner whose key is a lower bound is not added: this step's keys
+    // are kept (c's row; p's row is dropped) and it is resolved below.
     const bool hit = !fl1 && spec && v1 == pr;
     const bool fl2 = hit && (g1.index() & 1);
     const int v2 = hit && !fl2 ? g1.index() >> 1 : -1;
@@ -767,10 +797,10 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
         a.edge_v[o] = v2;
         a.edge_w[o] = gw;
       }
-      best[s] = hit ? b2[s] : b1[s];
+      bb[s] = hit ? b2[s] : b1[s];
       src[s] = hit ? s2[s] : s1[s];
+      fg[s] = hit ? f2[s] : f1[s];
     }
-    flg = hit ? fm2 : fm1;
     if (fl1 || fl2) {
       // The winner (or, after p was added, the runner-up) needs its exact
       // key before it can be added; the next round reads no row.

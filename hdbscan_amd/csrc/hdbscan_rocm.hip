@@ -759,7 +759,7 @@ int launch_gx_matrix(hipStream_t s, const Plan& P, const float* x, int64_t batch
       x, batch, (int)n, (int)d, nblk, op, norms, u2, M, lcnt, lval, ncnt, nlist, cap, status);
   if (int r = check_launch("gx_matrix_kernel")) return r;
   stage_mark(s, "gx_matrix_kernel");
-  gx_near_kernel<<<xcd_grid(batch, (cap + 255) / 256), 256, 0, s>>>(x, batch, (int)n, (int)d, cap, ncnt, nlist, u2,
+  gx_near_kernel<DK><<<xcd_grid(batch, (cap + 255) / 256), 256, 0, s>>>(x, batch, (int)n, (int)d, cap, ncnt, nlist, u2,
                                                                    M, lcnt, lval, status);
   return check_launch("gx_near_kernel");
 }
