@@ -331,6 +331,31 @@ template <> struct BitsOfReal<double> {
   __device__ static uint64_t get(double v) { return (uint64_t)__double_as_longlong(v); }
   __device__ static double put(uint64_t u) { return __longlong_as_double((long long)u); }
 };
+// Bit pattern of a non-negative weight (or +inf): unsigned order of the bits
+// is the order of the values, so max and < are single integer operations.
+template <typename Real> struct WBits;
+template <> struct WBits<float> {
+  using U = uint32_t;
+  __device__ static U of(float v) { return __float_as_uint(v); }
+  __device__ static bool finite(U u) { return (u & 0x7FFFFFFFu) < 0x7F800000u; }
+  __device__ static Cand<float> key(U w, int j) {
+    Cand<float> c;
+    c.k = ((uint64_t)w << 32) | (uint32_t)j;
+    return c;
+  }
+};
+template <> struct WBits<double> {
+  using U = uint64_t;
+  __device__ static U of(double v) { return (uint64_t)__double_as_longlong(v); }
+  __device__ static bool finite(U u) { return (u & 0x7FFFFFFFFFFFFFFFull) < 0x7FF0000000000000ull; }
+  __device__ static Cand<double> key(U w, int j) {
+    Cand<double> c;
+    c.w = w;
+    c.j = (uint32_t)j;
+    return c;
+  }
+};
+
 // Largest squared norm for which no squared distance of the window can
 // overflow (sum of squared differences <= 2 (|x|^2 + |y|^2) <= 4 * limit).
 template <typename Real> struct GramLimit;
@@ -756,15 +781,20 @@ __device__ __forceinline__ void prim_spec_window(const PrimArgs& a, int item) {
   if (tid == 0) sh_err = 0;
   if (!unit)
     for (int i = tid; i < n; i += T) cs[i] = cg[i];
-  Real best[VPT], cr[VPT];
-  int src[VPT];
+  // Keys, cores and weights as bit patterns (WBits): no divergent branch in
+  // the round, both rows read whole with clamped indices.
+  using WB = WBits<Real>;
+  using U = typename WB::U;
+  U bb[VPT], crb[VPT];
+  int src[VPT], jc[VPT];
   uint32_t vis = 0;
 #pragma unroll
   for (int s = 0; s < VPT; ++s) {
     const int j = tid + s * T;
-    best[s] = Real(INFINITY);
+    bb[s] = WB::of(Real(INFINITY));
     src[s] = 1;
-    cr[s] = j < n && !unit ? cg[j] : Real(0);
+    jc[s] = min(j, n - 1);
+    crb[s] = WB::of(j < n && !unit ? cg[j] : Real(0));
     if (j >= n) vis |= 1u << s;
   }
   if (tid == 0) vis |= 1u;  // vertex 0 starts the tree
@@ -774,62 +804,55 @@ __device__ __forceinline__ void prim_spec_window(const PrimArgs& a, int item) {
   const C none = C::make(rmax, 0);
   int cur = 0, pr = -1, par = 0, bad = 0;
   for (int step = 0; step < n - 1;) {
+    cur = __builtin_amdgcn_readfirstlane(cur);  // uniform: row addresses in scalar registers
+    pr = __builtin_amdgcn_readfirstlane(pr);
     // Speculate only with two or more unvisited vertices.
     const bool spec = pr >= 0 && n - 1 - step >= 2;
-    const Real ccur = unit ? Real(0) : cs[cur];
-    const Real cpr = spec && !unit ? cs[pr] : Real(0);
-    Real d1[VPT], d2[VPT];
+    const U ccur = unit ? U(0) : WB::of(cs[cur]);
+    const U cpr = spec && !unit ? WB::of(cs[pr]) : U(0);
+    const Real* row1 = dg + (size_t)cur * n;
+    const Real* row2 = dg + (size_t)(spec ? pr : cur) * n;
+    U d1[VPT], d2[VPT];
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
-      const int j = tid + s * T;
-      const bool un = !((vis >> s) & 1u);
-      // Rows are read only where a strictly smaller weight is possible; for
-      // p's row the test uses the keys before c's row (a superset).
-      const bool l1 = un && (unit || fmax(ccur, cr[s]) < best[s]);
-      const bool l2 = spec && un && j != pr && (unit || fmax(cpr, cr[s]) < best[s]);
-      d1[s] = l1 ? dg[(size_t)cur * n + j] : Real(0);
-      d2[s] = l2 ? dg[(size_t)pr * n + j] : Real(0);
+      d1[s] = WB::of(row1[jc[s]]);
+      d2[s] = spec ? WB::of(row2[jc[s]]) : U(0);
     }
-    Real b1[VPT], b2[VPT];
+    U b1[VPT], b2[VPT];
     int s1[VPT], s2[VPT];
     C k1a = none, k1b = none, k2a = none, k2b = none;
     int bad2 = 0;
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
       const int j = tid + s * T;
-      b1[s] = best[s];
-      s1[s] = src[s];
-      b2[s] = best[s];
-      s2[s] = src[s];
-      if (!((vis >> s) & 1u)) {
-        if (unit || fmax(ccur, cr[s]) < best[s]) {
-          bad |= !isfinite(d1[s]);
-          const Real w = unit ? d1[s] : fmax(fmax(ccur, cr[s]), d1[s]);
-          if (w < b1[s]) {
-            b1[s] = w;
-            s1[s] = cur;
-          }
-        }
-        // K': the two smallest predict the next round of a plain round; a
-        // speculative round needs only the smallest.
-        const C o1 = C::make(b1[s], j);
-        if (spec)
-          k1a = cand_pick(o1.less(k1a), o1, k1a);
-        else
-          cand_top2(k1a, k1b, o1);
-        if (spec && j != pr) {
-          b2[s] = b1[s];
-          s2[s] = s1[s];
-          if (unit || fmax(cpr, cr[s]) < b1[s]) {
-            bad2 |= !isfinite(d2[s]);
-            const Real w = unit ? d2[s] : fmax(fmax(cpr, cr[s]), d2[s]);
-            if (w < b2[s]) {
-              b2[s] = w;
-              s2[s] = pr;
-            }
-          }
-          cand_top2(k2a, k2b, C::make(b2[s], j));
-        }
+      const bool un = !((vis >> s) & 1u);
+      // Only a weight strictly below the key can change it:
+      // max(core_c, core_j) < key is needed before the entry matters.
+      const U m1 = max(ccur, crb[s]);
+      const bool l1 = un && m1 < bb[s];
+      bad |= l1 && !WB::finite(d1[s]);
+      const U w1 = max(m1, d1[s]);
+      const bool lt1 = l1 && w1 < bb[s];
+      b1[s] = lt1 ? w1 : bb[s];
+      s1[s] = lt1 ? cur : src[s];
+      // K': the two smallest predict the next round of a plain round; a
+      // speculative round needs only the smallest.
+      const C o1 = un ? WB::key(b1[s], j) : C::none();
+      if (spec) {
+        k1a = cand_pick(o1.less(k1a), o1, k1a);
+        const bool e2 = un && j != pr;
+        const U m2 = max(cpr, crb[s]);
+        const bool l2 = e2 && m2 < b1[s];
+        bad2 |= l2 && !WB::finite(d2[s]);
+        const U w2 = max(m2, d2[s]);
+        const bool lt2 = l2 && w2 < b1[s];
+        b2[s] = lt2 ? w2 : b1[s];
+        s2[s] = lt2 ? pr : s1[s];
+        cand_top2(k2a, k2b, e2 ? WB::key(b2[s], j) : C::none());
+      } else {
+        b2[s] = b1[s];
+        s2[s] = s1[s];
+        cand_top2(k1a, k1b, o1);
       }
     }
     // Per wave: the list that predicts (K'' when speculating, else K') to
@@ -883,7 +906,7 @@ __device__ __forceinline__ void prim_spec_window(const PrimArgs& a, int item) {
         a.edge_v[o] = v2;
         static_cast<Real*>(a.edge_w)[o] = gw;
       }
-      best[s] = hit ? b2[s] : b1[s];
+      bb[s] = hit ? b2[s] : b1[s];
       src[s] = hit ? s2[s] : s1[s];
     }
     if (hit) bad |= bad2;
