@@ -33,7 +33,9 @@
 //                     when the last entry that lowered it was a far one (a
 //                     flag bit).  If a step's argmin is flagged, its exact
 //                     key is computed first (resolution, below) and the
-//                     argmin is taken again.
+//                     argmin is taken again, in a round that reads no row
+//                     and speculates on the resolved vertex with the row
+//                     the resolution read.
 //
 // Resolution of vertex r: its row holds, for each visited c, the exact
 // distance or a lower bound lbd; for the latter an upper bound follows from
@@ -610,6 +612,10 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
   const C none = C::make(rmax, 0);
   int cur = 0, pr = -1, par = 0, bad = 0;
   bool rowless = false;
+  // Row of the vertex resolved last, kept for the round after its
+  // resolution, which speculates on that vertex without reading a row.
+  uint32_t rrow[VPT];
+  bool keep = false;
 
   // Exact key and source of unvisited vertex r over the visited vertices.
   // Far pairs whose lower weight is <= T are listed in LDS and evaluated one
@@ -628,9 +634,10 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
       wx[s] = INFINITY;
       kl[s] = INFINITY;
       cm[s] = unit ? 0.f : fmaxf(cr[s], cor);
+      rrow[s] = __float_as_uint(mr[jc[s]]);
       if (((vis >> s) & 1u) && c < n) {
         okm |= 1u << s;
-        const float v = mr[c];
+        const float v = __uint_as_float(rrow[s]);
         if (__float_as_int(v) < 0) {
           const float lbd = -v;
           kl[s] = fmaxf(cm[s], lbd);
@@ -701,7 +708,9 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
   for (int step = 0; step < n - 1;) {
     cur = __builtin_amdgcn_readfirstlane(cur);  // uniform: row addresses in scalar registers
     pr = __builtin_amdgcn_readfirstlane(pr);
-    const bool spec = !rowless && pr >= 0 && n - 1 - step >= 2;
+    // After a resolution: no row for c, and the resolved vertex's row (kept
+    // in registers) as p's.
+    const bool spec = (keep || (!rowless && pr >= 0)) && n - 1 - step >= 2;
     const uint32_t ccur = rowless || unit ? 0u : __float_as_uint(cs[cur]);
     const uint32_t cpr = spec && !unit ? __float_as_uint(cs[pr]) : 0u;
     // Both rows are read whole (clamped index past the window), all loads
@@ -712,7 +721,7 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
 #pragma unroll
     for (int s = 0; s < VPT; ++s) {
       d1[s] = rowless ? 0u : __float_as_uint(row1[jc[s]]);
-      d2[s] = spec ? __float_as_uint(row2[jc[s]]) : 0u;
+      d2[s] = spec ? (keep ? rrow[s] : __float_as_uint(row2[jc[s]])) : 0u;
     }
     uint32_t b1[VPT], b2[VPT], f1[VPT], f2[VPT];
     int s1[VPT], s2[VPT];
@@ -803,16 +812,20 @@ __global__ __launch_bounds__(1024) void prim_gx_kernel(GxPrimArgs a) {
     }
     if (fl1 || fl2) {
       // The winner (or, after p was added, the runner-up) needs its exact
-      // key before it can be added; the next round reads no row.
+      // key before it can be added; the next round reads no row and
+      // speculates on the resolved vertex with the row just read.
       step += fl2 ? 1 : 0;
-      resolve(fl1 ? v1 : g1.index() >> 1);
+      const int r = fl1 ? v1 : g1.index() >> 1;
+      resolve(r);
       rowless = true;
-      pr = -1;
+      pr = r;
+      keep = true;
       continue;
     }
     step += hit ? 2 : 1;
     cur = hit ? v2 : v1;
     rowless = false;
+    keep = false;
     pr = (!spec || hit) && g2.less(none) ? g2.index() >> 1 : -1;
   }
   if (bad) atomicOr(&sh_err, 1);
