@@ -103,9 +103,11 @@ __host__ __device__ constexpr size_t gx_operand_entries(int dk, int64_t batch, i
 typedef float gx_f32x4 __attribute__((ext_vector_type(4)));
 
 // Matrix stores (ordinary stores: non-temporal ones measured slower).
-__device__ __forceinline__ void gx_store(float* p, float v) { *p = v; }
+// Matrix entries are written once and read by the Prim much later: streaming
+// (non-temporal) stores keep them from evicting the MFMA operands from L2.
+__device__ __forceinline__ void gx_store(float* p, float v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void gx_store4(float* p, float a, float b, float c, float d) {
-  *reinterpret_cast<gx_f32x4*>(p) = gx_f32x4{a, b, c, d};
+  __builtin_nontemporal_store(gx_f32x4{a, b, c, d}, reinterpret_cast<gx_f32x4*>(p));
 }
 
 constexpr int kXcds = 8;
@@ -241,95 +243,130 @@ __device__ __forceinline__ void gx_append(int32_t* lcnt, float* lval, int64_t p,
   if (k < kGxCoreCap) lval[(size_t)p * kGxCoreCap + k] = v;
 }
 
-// 128 x 128 tiles of the upper triangle of window b's matrix (tile t of the
-// triangle in row order), 256 threads, each wave a 64 x 64 quarter as 4 x 4
-// MFMA blocks of 16 x 16.  Persistent: workgroup g takes tile ids g, g + G,
-// ... (G workgroups, a multiple of 8, so g keeps its XCD), so that the
-// stores of one tile drain while the next tile's operands load.  Far values are written as soon as they are
-// classified (the tile and its mirror); near pairs go to the wave's LDS list
-// and are evaluated exactly by the whole workgroup afterwards, so no address
-// is written twice.  Registers are held to four waves per SIMD so that the
-// exact evaluations of some tiles overlap the matrix products of others.
-// Tile ids: xcd_grid(batch, nt (nt + 1) / 2), nt = ceil(n / 128).
+// 128 x 128 tiles of the upper triangle of each window's matrix (tile t of
+// the triangle in row order), each quarter of 64 x 64 as 4 x 4 MFMA blocks of
+// 16 x 16: the three split-bf16 products accumulated on the lower norm terms
+// give lb (a lower bound of the reference's squared distance).  A pair is near
+// when lb <= max(U2_i, U2_j); far entries store minus a lower bound of the
+// distance in both halves, near pairs go to the window's list for
+// gx_near_kernel (pairs past the list's capacity are evaluated here).
+// Off-diagonal tiles inside the window store every entry, a near one with its
+// lower bound, which the exact value replaces later; other tiles store far
+// entries only.  Tiles holding a point whose squared norm exceeds 2^100 store
+// exact distances.
+//
+// Wave-specialised (512 threads, one workgroup per CU): waves 0-3 (MFMA
+// waves) compute tile k's quarters while waves 4-7 (store waves) classify
+// and store tile k - 1; the accumulators pass through LDS
+// (lane-major, one 64 x 64 quarter per wave pair) between two barriers per
+// tile.  A wave's memory counter is in order and counts stores too, so a
+// wave that both stores and loads waits for its stores at its next load;
+// here the MFMA waves never store to device memory and the store waves load
+// nothing from it (the near-list reservation is made before the tile's
+// stores, so it waits only for stores a tile older).  Per iteration k:
+//   A: MFMA waves: products of tile k;  store waves: tile k - 1 from LDS;
+//   barrier;
+//   B: MFMA waves: accumulators of tile k to LDS, norm terms and U2 of
+//      tile k + 1 (parity (k + 1) & 1), its exact-tile flag (k + 1) % 3;
+//   barrier.
+// Tile ids: workgroup g takes g, g + G, ... (G a multiple of 8 keeps g on
+// its XCD).
 template <int DK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-void gx_matrix_kernel(const float* x, int64_t batch, int n, int d, int nblk, const gram_s16x8* op,
-                      const float* norms, const float* U2, float* M, int32_t* lcnt, float* lval,
-                      int32_t* ncnt, uint32_t* nlist, int cap, int32_t* status) {
+__global__ __launch_bounds__(512) void gx_matrix_kernel(const float* x, int64_t batch, int n, int d, int nblk,
+                                                           const gram_s16x8* op, const float* norms, const float* U2,
+                                                           float* M, int32_t* lcnt, float* lval, int32_t* ncnt,
+                                                           uint32_t* nlist, int cap, int32_t* status) {
   constexpr int NQ1 = DK / 32, NQ2 = 2 * NQ1, NQ3 = 3 * NQ1;
   constexpr float kLower = gram_c(DK);
-  __shared__ float s_n[2][kGxTile];
-  __shared__ float s_u[2][kGxTile];
+  __shared__ float s_acc[4][64 * 64];
+  __shared__ float s_n[2][2][kGxTile];
+  __shared__ float s_u[2][2][kGxTile];
+  __shared__ int s_exact[3];
   __shared__ uint16_t s_pos[4][kGxTile * kGxTile / 4];
   const int nt = (n + kGxTile - 1) / kGxTile;
   const int per = nt * (nt + 1) / 2;
   const int64_t ids = ((batch + kXcds - 1) / kXcds) * kXcds * per;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool mw = wave < 4;
+  const int q = wave & 3;
+  const int wr = (q >> 1) * 64, wc = (q & 1) * 64;
+  const int g = lane >> 4, l16 = lane & 15;
+  const int qt = tid & 255;  // thread index within its four waves
   int bad = 0;
-  for (int64_t L = blockIdx.x; L < ids; L += gridDim.x) {
-    int b, t;
-    if (!xcd_item(L, batch, per, b, t)) continue;  // uniform per workgroup
+  // Tile k of this workgroup: false past the last tile id or for a window
+  // index past the batch (uniform).
+  auto tile = [&](int64_t k, int& b, int& r0, int& c0, bool& diag) -> bool {
+    const int64_t L = blockIdx.x + k * gridDim.x;
+    int t;
+    if (L >= ids || !xcd_item(L, batch, per, b, t)) return false;
     int ti = 0;
     while (t >= nt - ti) {
       t -= nt - ti;
       ++ti;
     }
-    const int tj = ti + t;
-    const int r0 = ti * kGxTile, c0 = tj * kGxTile;
-    const bool diag = ti == tj;
+    r0 = ti * kGxTile;
+    c0 = (ti + t) * kGxTile;
+    diag = t == 0;
+    return true;
+  };
+  // MFMA waves: the lower norm terms n (1 - c) - 2^-110 and U2 of tile k's
+  // rows and columns; a point outside the bounded range marks the tile exact.
+  auto load_terms = [&](int64_t k) {
+    int b, r0, c0;
+    bool diag;
+    if (!tile(k, b, r0, c0, diag)) return;
+    const int side = qt >> 7, kk = qt & (kGxTile - 1);
+    const int p = (side ? c0 : r0) + kk;
+    const bool in = p < n;
     const int64_t base = (int64_t)b * n;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int g = lane >> 4, l16 = lane & 15;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-    float* mb = M + (size_t)base * n;
-    const float* xb = x + (size_t)base * d;
-    float npt;
-    {
-      const int side = tid >> 7, k = tid & (kGxTile - 1);
-      const int p = (side ? c0 : r0) + k;
-      const bool in = p < n;
-      npt = in ? norms[base + p] : 0.f;
-      // The lower norm term n (1 - c) - 2^-110 of hdb_gram.h: the three
-      // split-bf16 products accumulated on t_i + t_j give a lower bound of
-      // the reference's squared distance.
-      s_n[side][k] = fmaf(npt, -kLower, npt) - kGramEh;
-      s_u[side][k] = in ? U2[base + p] : -INFINITY;
-    }
-    // A point outside the bounded range sends the tile to the exact path.
-    const bool exact_tile = __syncthreads_or(!(npt <= kGramNormLimit));
-    if (!exact_tile) {
-      int cnt = 0;
-      // The lower-left quarter of a diagonal tile holds no pair i < j.
-      if (!(diag && wr > wc)) {
-        gram_f32x4 acc[4][4];
+    const float npt = in ? norms[base + p] : 0.f;
+    s_n[k & 1][side][kk] = fmaf(npt, -kLower, npt) - kGramEh;
+    s_u[k & 1][side][kk] = in ? U2[base + p] : -INFINITY;
+    if (!(npt <= kGramNormLimit)) atomicOr(&s_exact[k % 3], 1);
+  };
+  if (tid == 0) s_exact[0] = s_exact[1] = s_exact[2] = 0;
+  __syncthreads();
+  if (mw) load_terms(0);
+  __syncthreads();
+  gram_f32x4 acc[4][4];
+  for (int64_t k = 0;; ++k) {
+    const bool have_k = blockIdx.x + k * gridDim.x < ids;
+    if (!have_k && (k == 0 || blockIdx.x + (k - 1) * gridDim.x >= ids)) break;  // uniform
+    bool prod = false;  // MFMA waves: this quarter of tile k was computed
+    if (mw) {
+      if (tid == 0) s_exact[(k + 1) % 3] = 0;
+      int b, r0, c0;
+      bool diag;
+      if (have_k && tile(k, b, r0, c0, diag) && !s_exact[k % 3] && !(diag && wr > wc)) {
+        prod = true;
+        const float (*sn)[kGxTile] = s_n[k & 1];
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
           for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-              acc[rb][cb][r] = -0.5f * (s_n[0][wr + 16 * rb + 4 * g + r] + s_n[1][wc + 16 * cb + l16]);
-        // Operand blocks of the wave's rows and columns (blocks past the window
-        // are clamped to its last one; their results are not stored).
+              acc[rb][cb][r] = -0.5f * (sn[0][wr + 16 * rb + 4 * g + r] + sn[1][wc + 16 * cb + l16]);
         const gram_s16x8* opl = op + lane;
         size_t aoff[4], boff[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          aoff[k] = (size_t)(b * nblk + min((r0 + wr) / 16 + k, nblk - 1)) * NQ2 * 64;
-          boff[k] = (size_t)(b * nblk + min((c0 + wc) / 16 + k, nblk - 1)) * NQ2 * 64;
+        for (int kk = 0; kk < 4; ++kk) {
+          aoff[kk] = (size_t)(b * nblk + min((r0 + wr) / 16 + kk, nblk - 1)) * NQ2 * 64;
+          boff[kk] = (size_t)(b * nblk + min((c0 + wc) / 16 + kk, nblk - 1)) * NQ2 * 64;
         }
-        // K-steps: A = hi | hi | lo, B = hi | lo | hi (products hi.hi, hi.lo,
-        // lo.hi).  Unrolled by two only: deeper unrolling prefetches operands
-        // past the register budget of four waves per SIMD.
+        // K-steps: A = hi | hi | lo, B = hi | lo | hi (products hi.hi,
+        // hi.lo, lo.hi); blocks past the window are clamped to its last one
+        // (their results are not stored).
 #pragma unroll 2
-        for (int q = 0; q < NQ3; ++q) {
-          const int seg = q / NQ1, qq = q % NQ1;
+        for (int qq3 = 0; qq3 < NQ3; ++qq3) {
+          const int seg = qq3 / NQ1, qq = qq3 % NQ1;
           const int qa = seg == 2 ? NQ1 + qq : qq, qb = seg == 1 ? NQ1 + qq : qq;
           gram_bf16x8 av[4], bv[4];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            av[k] = __builtin_bit_cast(gram_bf16x8, opl[aoff[k] + (size_t)qa * 64]);
-            bv[k] = __builtin_bit_cast(gram_bf16x8, opl[boff[k] + (size_t)qb * 64]);
+          for (int kk = 0; kk < 4; ++kk) {
+            av[kk] = __builtin_bit_cast(gram_bf16x8, opl[aoff[kk] + (size_t)qa * 64]);
+            bv[kk] = __builtin_bit_cast(gram_bf16x8, opl[boff[kk] + (size_t)qb * 64]);
           }
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb)
@@ -337,108 +374,167 @@ void gx_matrix_kernel(const float* x, int64_t batch, int n, int d, int nblk, con
             for (int cb = 0; cb < 4; ++cb)
               acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rb], bv[cb], acc[rb][cb], 0, 0, 0);
         }
-        const bool vec = !diag && (n & 3) == 0;
+      }
+    } else if (k > 0) {
+      int b, r0, c0;
+      bool diag;
+      if (tile(k - 1, b, r0, c0, diag)) {
+        const int64_t base = (int64_t)b * n;
+        float* mb = M + (size_t)base * n;
+        const float* xb = x + (size_t)base * d;
+        const float (*su)[kGxTile] = s_u[(k - 1) & 1];
+        if (s_exact[(k - 1) % 3]) {
+          // Every pair of the tile exactly (a point beyond the bounded range).
+          for (int idx = qt; idx < kGxTile * kGxTile; idx += 256) {
+            const int li = idx >> 7, lj = idx & 127;
+            const int i = r0 + li, j = c0 + lj;
+            if (i >= n || j >= n || (diag && i > j)) continue;
+            if (i == j) {
+              gx_store(mb + (size_t)i * n + j, 0.f);
+              continue;
+            }
+            const float sq = exact_sq(xb + (size_t)i * d, xb + (size_t)j * d, d);
+            bad |= !isfinite(sq);
+            const float dv = sqrtf(sq);
+            gx_store(mb + (size_t)i * n + j, dv);
+            gx_store(mb + (size_t)j * n + i, dv);
+            if (sq <= su[0][li]) gx_append(lcnt, lval, base + i, dv);
+            if (sq <= su[1][lj]) gx_append(lcnt, lval, base + j, dv);
+          }
+        } else if (!(diag && wr > wc)) {
+          // The lower-left quarter of a diagonal tile holds no pair i < j.
+          const float* sa = s_acc[q];
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-          const int li0 = wr + 16 * rb + 4 * g, i0 = r0 + li0;
-          float ur[4];
+          for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) ur[r] = s_u[0][li0 + r];
+            for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-          for (int cb = 0; cb < 4; ++cb) {
-            const int lj = wc + 16 * cb + l16, j = c0 + lj;
-            const float uc = s_u[1][lj];
-            float v[4];
-            bool nr[4];
+              for (int r = 0; r < 4; ++r) acc[rb][cb][r] = sa[((rb * 4 + cb) * 4 + r) * 64 + lane];
+          const bool vec = !diag && (n & 3) == 0;
+          const bool interior = vec && r0 + kGxTile <= n && c0 + kGxTile <= n;
+          // Classification: acc becomes the stored value (minus the lower
+          // bound), near entries go to the wave's list.
+          uint64_t nrm = 0;
+          int cnt = 0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int i = i0 + r;
-              const bool valid = (i < n) & (j < n) & (!diag | (i < j));
-              const float lbs = -2.f * acc[rb][cb][r];
-              nr[r] = valid & (lbs <= fmaxf(ur[r], uc));
-              v[r] = lbs > 0.f ? -(__builtin_amdgcn_sqrtf(lbs) * kGxDown) : -0.f;
-              const uint64_t bal = __ballot(nr[r]);
-              if (bal) {
-                const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                if (nr[r]) s_pos[wave][slot] = (uint16_t)(((li0 + r) << 7) | lj);
-                cnt += __popcll(bal);
+          for (int rb = 0; rb < 4; ++rb) {
+            const int li0 = wr + 16 * rb + 4 * g, i0 = r0 + li0;
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+              const int lj = wc + 16 * cb + l16, j = c0 + lj;
+              const float uc = su[1][lj];
+              bool nr[4];
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int i = i0 + r;
+                const bool valid = interior || ((i < n) & (j < n) & (!diag | (i < j)));
+                const float lbs = -2.f * acc[rb][cb][r];
+                nr[r] = valid & (lbs <= fmaxf(su[0][li0 + r], uc));
+                // sqrt(+0) = +0 for lbs <= 0: the bound -0.
+                acc[rb][cb][r] = __builtin_amdgcn_sqrtf(fmaxf(lbs, 0.f)) * -kGxDown;
+                nrm |= (uint64_t)nr[r] << ((rb * 4 + cb) * 4 + r);
               }
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int i = i0 + r;
-              if (i < n && j < n && (!diag || i <= j) && !nr[r]) gx_store(mb + (size_t)i * n + j, i == j ? 0.f : v[r]);
-            }
-            // Mirror: row j, columns i0 .. i0 + 3.
-            if (j < n) {
-              if (vec && i0 + 3 < n && !(nr[0] || nr[1] || nr[2] || nr[3])) {
-                gx_store4(mb + (size_t)j * n + i0, v[0], v[1], v[2], v[3]);
-              } else {
+              if (__ballot(nr[0] | nr[1] | nr[2] | nr[3])) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                  const int i = i0 + r;
-                  if (i < n && (!diag || i < j) && !nr[r]) gx_store(mb + (size_t)j * n + i, v[r]);
+                  const uint64_t bal = __ballot(nr[r]);
+                  const int slot = cnt + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                  if (nr[r]) s_pos[q][slot] = (uint16_t)(((li0 + r) << 7) | lj);
+                  cnt += __popcll(bal);
                 }
               }
             }
           }
+          // The wave's near pairs get their place in the window's list
+          // before this tile's stores are issued.
+          int first = 0;
+          if (lane == 0 && cnt) first = atomicAdd(ncnt + b, cnt);
+          first = __shfl(first, 0);
+          if (interior) {
+            // Every entry, near ones with their lower bound (replaced later).
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+              const int i0 = r0 + wr + 16 * rb + 4 * g;
+#pragma unroll
+              for (int cb = 0; cb < 4; ++cb) {
+                const int j = c0 + wc + 16 * cb + l16;
+                const gram_f32x4 v = acc[rb][cb];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) gx_store(mb + (size_t)(i0 + r) * n + j, v[r]);
+                gx_store4(mb + (size_t)j * n + i0, v[0], v[1], v[2], v[3]);
+              }
+            }
+          } else {
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+              const int i0 = r0 + wr + 16 * rb + 4 * g;
+#pragma unroll
+              for (int cb = 0; cb < 4; ++cb) {
+                const int j = c0 + wc + 16 * cb + l16;
+                const gram_f32x4 v = acc[rb][cb];
+                const uint32_t m4 = (uint32_t)(nrm >> ((rb * 4 + cb) * 4)) & 15u;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                  const int i = i0 + r;
+                  if (i < n && j < n && (!diag || i <= j) && !((m4 >> r) & 1u))
+                    gx_store(mb + (size_t)i * n + j, i == j ? 0.f : v[r]);
+                }
+                // Mirror: row j, columns i0 .. i0 + 3.
+                if (j < n) {
+                  if (vec && i0 + 3 < n && m4 == 0) {
+                    gx_store4(mb + (size_t)j * n + i0, v[0], v[1], v[2], v[3]);
+                  } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                      const int i = i0 + r;
+                      if (i < n && (!diag || i < j) && !((m4 >> r) & 1u)) gx_store(mb + (size_t)j * n + i, v[r]);
+                    }
+                  }
+                }
+              }
+            }
+          }
+          // Near pairs to the list for gx_near_kernel; pairs past its
+          // capacity exactly here (after this wave's stores of the tile).
+          const int room = max(0, min(cnt, cap - first));
+          for (int e = lane; e < cnt; e += 64) {
+            const int pos = s_pos[q][e];
+            const int li = pos >> 7, lj = pos & 127;
+            const int i = r0 + li, j = c0 + lj;
+            if (e < room) {
+              nlist[(size_t)b * cap + first + e] = ((uint32_t)i << 12) | (uint32_t)j;
+              continue;
+            }
+            const float sq = exact_sq(xb + (size_t)i * d, xb + (size_t)j * d, d);
+            bad |= !isfinite(sq);
+            const float dv = sqrtf(sq);
+            gx_store(mb + (size_t)i * n + j, dv);
+            gx_store(mb + (size_t)j * n + i, dv);
+            if (sq <= su[0][li]) gx_append(lcnt, lval, base + i, dv);
+            if (sq <= su[1][lj]) gx_append(lcnt, lval, base + j, dv);
+          }
         }
-      }
-      __syncthreads();
-      // The wave's near pairs go to the window's list (one reservation per
-      // wave) for gx_near_kernel; pairs past the list's capacity are
-      // evaluated here.
-      int first = 0;
-      if (lane == 0 && cnt) first = atomicAdd(ncnt + b, cnt);
-      first = __shfl(first, 0);
-      const int room = max(0, min(cnt, cap - first));
-      for (int e = lane; e < cnt; e += 64) {
-        const int pos = s_pos[wave][e];
-        const int li = pos >> 7, lj = pos & 127;
-        const int i = r0 + li, j = c0 + lj;
-        if (e < room) {
-          nlist[(size_t)b * cap + first + e] = ((uint32_t)i << 12) | (uint32_t)j;
-          continue;
-        }
-        const float sq = exact_sq(xb + (size_t)i * d, xb + (size_t)j * d, d);
-        bad |= !isfinite(sq);
-        const float dv = sqrtf(sq);
-        gx_store(mb + (size_t)i * n + j, dv);
-        gx_store(mb + (size_t)j * n + i, dv);
-        if (sq <= s_u[0][li]) gx_append(lcnt, lval, base + i, dv);
-        if (sq <= s_u[1][lj]) gx_append(lcnt, lval, base + j, dv);
-      }
-    } else {
-      // Every pair of the tile exactly (a point beyond the bounded norm range).
-      for (int idx = tid; idx < kGxTile * kGxTile; idx += 256) {
-        const int li = idx >> 7, lj = idx & 127;
-        const int i = r0 + li, j = c0 + lj;
-        if (i >= n || j >= n || (diag && i > j)) continue;
-        if (i == j) {
-          gx_store(mb + (size_t)i * n + j, 0.f);
-          continue;
-        }
-        const float sq = exact_sq(xb + (size_t)i * d, xb + (size_t)j * d, d);
-        bad |= !isfinite(sq);
-        const float dv = sqrtf(sq);
-        gx_store(mb + (size_t)i * n + j, dv);
-        gx_store(mb + (size_t)j * n + i, dv);
-        if (sq <= s_u[0][li]) gx_append(lcnt, lval, base + i, dv);
-        if (sq <= s_u[1][lj]) gx_append(lcnt, lval, base + j, dv);
       }
     }
-    __syncthreads();  // the next tile rewrites the LDS arrays
+    __syncthreads();
+    if (mw) {
+      if (prod) {
+        float* sa = s_acc[q];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sa[((rb * 4 + cb) * 4 + r) * 64 + lane] = acc[rb][cb][r];
+      }
+      load_terms(k + 1);
+    }
+    __syncthreads();
   }
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
 }
 
-// Exact distances of the listed near pairs (coordinate order), one lane per
-// pair, written to both halves; core-list appends as in gx_matrix_kernel.
-// One workgroup per (window, chunk of 256 entries), window b on XCD b mod 8
-// and in window order there, so that a window's points stay in that XCD's
-// L2 while its pairs are evaluated; chunks past a window's count exit.
-// Grid: xcd_grid(batch, ceil(cap / 256)).
 // Exact coordinate-order squared distance of rows xi and xj for d <= DK:
 // the loads of 64 coordinates at a time are all issued before the first add
 // (the row pairs of one wave are scattered, so each phase costs one memory

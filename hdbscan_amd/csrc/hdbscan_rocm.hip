@@ -753,9 +753,10 @@ int launch_gx_matrix(hipStream_t s, const Plan& P, const float* x, int64_t batch
   int32_t* ncnt = reinterpret_cast<int32_t*>(ws + P.off_gncnt);
   uint32_t* nlist = reinterpret_cast<uint32_t*>(ws + P.off_gnlist);
   const int cap = kGxNearPerPoint * (int)n;
-  // Persistent: four workgroups per CU (a multiple of 8 keeps each on its XCD).
+  // Persistent: one wave-specialised workgroup per CU (a multiple of 8 keeps
+  // each on its XCD).
   const unsigned tiles = xcd_grid(batch, (int)(nt * (nt + 1) / 2));
-  gx_matrix_kernel<DK><<<std::min<unsigned>(tiles, 4u * (unsigned)((device_cus() + 7) / 8 * 8)), 256, 0, s>>>(
+  gx_matrix_kernel<DK><<<std::min<unsigned>(tiles, (unsigned)((device_cus() + 7) / 8 * 8)), 512, 0, s>>>(
       x, batch, (int)n, (int)d, nblk, op, norms, u2, M, lcnt, lval, ncnt, nlist, cap, status);
   if (int r = check_launch("gx_matrix_kernel")) return r;
   stage_mark(s, "gx_matrix_kernel");

@@ -48,6 +48,10 @@ def _windows():
         "offset-1e3-256x96": base[:256, :96] + 1e3,
         "duplicates-320x80": np.repeat(base[:40, :80], 8, axis=0),
         "all-equal-200x65": np.ones((200, 65)),
+        # Every pair near: the list overflows and interior tiles hold near
+        # pairs evaluated by the matrix kernel itself.
+        "all-equal-300x64": np.full((300, 64), 3.0),
+        "clumps-512x32": base[:4, :32][np.arange(512) % 4] + 0.0,
         "huge-in-256x72": base[:256, :72] * 2.0 ** 40,
         "huge-out-256x72": base[:256, :72] * 2.0 ** 52,
         "tiny-256x72": base[:256, :72] * 2.0 ** -60,
