@@ -77,7 +77,7 @@
 
 namespace hdb {
 
-constexpr int kGxTile = 128;       // matrix tile side (one workgroup of 256 threads)
+constexpr int kGxTile = 128;       // matrix tile side
 constexpr int kGxCoreCap = 64;     // core-list entries per point before the row is scanned
 constexpr int kGxNearPerPoint = 32;  // near-pair list capacity per window: 32 entries per point
 constexpr float kGxPassA = 0.008f;                    // hi.hi-only extra relative to S
@@ -94,15 +94,8 @@ __host__ __device__ constexpr size_t gx_operand_entries(int dk, int64_t batch, i
   return (size_t)batch * (size_t)((n + 15) / 16) * (size_t)(2 * dk / 32) * 64;
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup L to XCD
-// L mod 8), each XCD with its own L2.  The kernels that reread a window's
-// operands map the workgroups of window b to XCD b mod 8, in window order:
-// L = 8 (per * (b / 8) + item) + b mod 8 for the item-th of `per`
-// workgroups of window b.  Grid: 8 * ceil(batch / 8) * per workgroups; the
-// ones past the batch exit.
 typedef float gx_f32x4 __attribute__((ext_vector_type(4)));
 
-// Matrix stores (ordinary stores: non-temporal ones measured slower).
 // Matrix entries are written once and read by the Prim much later: streaming
 // (non-temporal) stores keep them from evicting the MFMA operands from L2.
 __device__ __forceinline__ void gx_store(float* p, float v) { __builtin_nontemporal_store(v, p); }
@@ -110,6 +103,12 @@ __device__ __forceinline__ void gx_store4(float* p, float a, float b, float c, f
   __builtin_nontemporal_store(gx_f32x4{a, b, c, d}, reinterpret_cast<gx_f32x4*>(p));
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup L to XCD
+// L mod 8), each XCD with its own L2.  The kernels that reread a window's
+// operands map the workgroups of window b to XCD b mod 8, in window order:
+// L = 8 (per * (b / 8) + item) + b mod 8 for the item-th of `per`
+// workgroups of window b.  Grid: 8 * ceil(batch / 8) * per workgroups; the
+// ones past the batch exit.
 constexpr int kXcds = 8;
 __device__ __forceinline__ bool xcd_item(int64_t L, int64_t batch, int per, int& b, int& item) {
   const int64_t k = L / kXcds;
@@ -262,8 +261,9 @@ __device__ __forceinline__ void gx_append(int32_t* lcnt, float* lval, int64_t p,
 // tile.  A wave's memory counter is in order and counts stores too, so a
 // wave that both stores and loads waits for its stores at its next load;
 // here the MFMA waves never store to device memory and the store waves load
-// nothing from it (the near-list reservation is made before the tile's
-// stores, so it waits only for stores a tile older).  Per iteration k:
+// from it only on the rare exact paths (exact tiles, pairs past the list's
+// capacity); their near-list reservation is made before the tile's stores,
+// so it waits only for stores a tile older.  Per iteration k:
 //   A: MFMA waves: products of tile k;  store waves: tile k - 1 from LDS;
 //   barrier;
 //   B: MFMA waves: accumulators of tile k to LDS, norm terms and U2 of
