@@ -154,6 +154,24 @@ def test_bounded_matrix_entries_are_exact_or_certified(name, ms):
     assert (np.diag(m).view(np.uint32) == 0).all()
 
 
+def test_bounded_matrix_entries_batch_and_odd_sizes():
+    """hdb_bounded_distances over a batch of windows (per-window matrices
+    and lists) whose size is not a multiple of 4 (no 16-byte mirror stores)
+    and spans partial tiles: every entry exact or a certified lower bound."""
+    rng = np.random.default_rng(11)
+    x = np.stack([_embed(rng, 389, 96, 4 + b, 0.3, 0.04) for b in range(5)]).astype(np.float32)
+    got, status = G.bounded_distances(x, 6)
+    assert status == 0
+    for b in range(len(x)):
+        m = got[b]
+        exact = oracle.distances(x[b], "float32")
+        far = np.signbit(m)
+        assert (~far).sum() > len(m)  # near pairs beyond the diagonal
+        np.testing.assert_array_equal(m[~far].view(np.uint32), exact[~far].view(np.uint32), err_msg=f"window {b}")
+        assert (-m[far] <= exact[far]).all(), b
+        np.testing.assert_array_equal(m.view(np.uint32), m.T.view(np.uint32))
+
+
 @pytest.mark.parametrize("value,bit", [(np.nan, 2), (np.inf, 2), (-np.inf, 2), (1e20, 1)])
 def test_bounded_matrix_status(value, bit):
     """Non-finite input (bit 2, from the prep kernel) and a distance that
