@@ -205,7 +205,10 @@ int hdb_ragged_set_tuning(const int32_t *bucket_flags, int count);
 
 /* Graph and tree stages from a given distance matrix [batch][n][n] and core
  * distances [batch][n] (core may be NULL when unit_core != 0, meaning every
- * mutual-reachability weight equals the distance).  When export_ints /
+ * mutual-reachability weight equals the distance).  Distances and core
+ * distances must be +0 or positive (weights are compared as IEEE bit
+ * patterns, whose order is the values' order only there), as the
+ * reference's distance kernel produces them.  When export_ints /
  * export_reals are non-NULL every intermediate is written per window:
  *   export_ints  [batch][10][n] int32: edge_u, edge_v, order, left, right,
  *                sizes, parent, selected, point_parent, {next_label, ...}
