@@ -59,6 +59,8 @@ def _windows():
         "three-3x90": base[:3, :90],
         "small-64x128": base[:64, :128],
         "d33-300x33": base[:300, :33],
+        "d32-260x32": base[:260, :32],
+        "d20-777x20": np.concatenate([rng.normal(c, 0.5, (111, 20)) for c in rng.uniform(-4, 4, (7, 20))]),
     }
     return {k: v.astype(np.float32) for k, v in out.items()}
 
