@@ -42,6 +42,7 @@ def _windows():
         "embed-300x128": _embed(rng, 300, 128, 6, 0.3, 0.03),
         "embed-1000x100": _embed(rng, 1000, 100, 9, 0.4, 0.05),
         "embed-2048x128": _embed(rng, 2048, 128, 16, 0.35, 0.03),
+        "embed-4096x96": _embed(rng, 4096, 96, 24, 0.4, 0.02),
         "blobs-517x40": np.concatenate([rng.normal(c, 1.0, (47, 40)) for c in rng.uniform(-6, 6, (11, 40))]),
         "blobs-700x200": np.concatenate([rng.normal(c, 0.7, (70, 200)) for c in rng.uniform(-3, 3, (10, 200))]),
         "grid-ties-400x70": rng.integers(-2, 3, (400, 70)) / 4.0,
