@@ -15,6 +15,7 @@ FLAG_PREFILTER = 32
 FLAG_PLAIN = 64
 FLAG_FILTER = 128
 FLAG_ONE_STEP = 65536
+FLAG_GRAM_CHUNKED = 131072
 FLAG_GRAM_GRAPH = 262144
 FLAG_NO_GRAPH = 524288
 FLAG_SPLIT_TREE = 1048576

@@ -405,6 +405,325 @@ __global__ __launch_bounds__(gram_max_threads<DK>()) void core_gram_kernel(Windo
   }
 }
 
+// Per-window form of both passes for windows of at most kWinMaxN points and
+// at most 16 dimensions (padded to 16).  One workgroup owns one window and
+// computes the bf16 pieces itself (no prep kernel).  Every column's B
+// operand is written into LDS once, in MFMA fragment order: fragment q of
+// 16-column block cb is 64 lanes x 16 bytes, lane l's eight K entries at
+// byte 16 l, so each operand read is one contiguous, conflict-free 16-byte
+// read per lane.  Each wave then runs both passes over all columns for its
+// 64 rows (four 16-row sets) with no barrier between column blocks (the
+// chunked kernel re-reads every column chunk per row block, behind two
+// barriers per chunk).
+//
+// The norm terms ride in K slots 48..53 of the second MFMA, which the
+// three 16-wide segments leave free: A holds the row term as three bf16
+// pieces in slots 48..50 and 1 in 51..53, B holds 1 in 48..50 and the
+// column term's pieces in 51..53.  The accumulator starts at zero, so no
+// VALU addition per tile forms the initial value.  Pieces of v: p1 =
+// bf16(v), p2 = bf16(v - p1), p3 = bf16(v - p1 - p2); each difference is
+// exact in float32 (it is v's low-order bits) and |v - p1 - p2 - p3| <=
+// 2^-24 |v|.  Bound: against the chunked form the initial-value rounding
+// disappears and six multiply-adds of magnitude <= 2.1 S (2u each) plus the
+// piece residuals (u S per term) appear, about 27 u S; the constant
+// gram_c(16 + 4) adds 64 u S.  Columns past the window carry a +inf term
+// (pieces +inf, 0, 0; the A side multiplies them by 1), so their bounds are
+// +inf: never among the smallest, never a candidate.
+constexpr int kWinDK = 16;
+constexpr int kWinMaxN = 1024;
+// Norm-term constant of the per-window form.
+constexpr float kWinC = gram_c(kWinDK + 4);
+
+// LDS bytes of core_window_kernel for windows of up to n points: two 1 KiB
+// fragments per 16-column block, then a candidate counter per row.
+HDB_HD inline uint32_t gram_window_lds_bytes(int n) { return (uint32_t)((n + 15) / 16) * 2048u + 4u * (uint32_t)n; }
+
+// The three bf16 pieces of v (finite) as bit patterns.
+__device__ __forceinline__ void bf16_pieces(float v, uint32_t& p1, uint32_t& p2, uint32_t& p3) {
+  p1 = bf16_bits(v);
+  const float r1 = v - bf16_value(p1);
+  p2 = bf16_bits(r1);
+  p3 = bf16_bits(r1 - bf16_value(p2));
+}
+
+// Eight K entries (slots 48..55) carrying a norm term: the term's pieces
+// first and three ones after them (row side), or the ones first (column
+// side).  inf: a column past the window.
+__device__ __forceinline__ gram_s16x8 norm_slots(float v, bool row_side, bool inf) {
+  uint32_t p1, p2, p3;
+  if (inf) {
+    p1 = 0x7F80u;  // +inf
+    p2 = p3 = 0;
+  } else {
+    bf16_pieces(v, p1, p2, p3);
+  }
+  constexpr short one = 0x3F80;  // bf16 1.0
+  gram_s16x8 s = {};
+  if (row_side) {
+    s[0] = (short)p1; s[1] = (short)p2; s[2] = (short)p3;
+    s[3] = one; s[4] = one; s[5] = one;
+  } else {
+    s[0] = one; s[1] = one; s[2] = one;
+    s[3] = (short)p1; s[4] = (short)p2; s[5] = (short)p3;
+  }
+  return s;
+}
+
+// Loads point p's DK padded coordinates (zeros past d), returns its squared
+// norm (FMA chain in coordinate order, as gram_prep_kernel) and ORs 1 into
+// bad for a non-finite coordinate.
+__device__ __forceinline__ float window_point(const float* xp, int d, float (&v)[kWinDK], int& bad) {
+  float sq = 0.f;
+#pragma unroll
+  for (int k = 0; k < kWinDK; ++k) {
+    v[k] = k < d ? xp[k] : 0.f;
+    bad |= !isfinite(v[k]);
+    sq = fmaf(v[k], v[k], sq);
+  }
+  return sq;
+}
+
+// Outputs as core_gram_kernel: cand[point][kGramCap] column indices and
+// ncand[point] (> kGramCap: recompute the row exactly; the whole window
+// when some squared norm exceeds 2^100 or is not finite).  Grid: one
+// workgroup per window of the launch, blockDim.x >= the largest window
+// rounded up to 64, LDS gram_window_lds_bytes(largest window).
+//
+// The column fragment is the MFMA's first operand and the row fragment the
+// second, so accumulator register r of lane l holds column 4 (l >> 4) + r of
+// the block and row l & 15 of the set: each lane owns one row per set and
+// sees four of its columns per block (the row's 16 columns of a block are
+// spread over the lanes l & 15 + 16 k, k = 0..3).
+//   Pass A: each lane keeps the M smallest upper bounds of its columns; U =
+//           the ms-th smallest of the 4 M values of the row's four lanes (a
+//           subset of the row, so U >= the ms-th smallest upper bound >= the
+//           exact statistic).  M = 2 for ms <= 8, 4 for ms <= 16: 4 M >= ms.
+//   Pass B: a column is a candidate when its lower bound is <= U.  Each
+//           lane ORs the test results of eight blocks into one mask word
+//           (bit 4 c + r: block c of the eight, register r) and only set
+//           bits are expanded: a slot from the row's LDS counter (atomic
+//           add), the column written to the row's list when the slot is
+//           below kGramCap.  The counter is the row's candidate count.
+// M: values kept per lane in pass A (2, 4).
+template <int M>
+__global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windows win, int d, int ms,
+                                                           uint16_t* cand, int32_t* ncand, int32_t* status) {
+  constexpr int DK = kWinDK, CAP = kGramCap, RS = 4;
+  extern __shared__ __align__(16) unsigned char lds[];
+  gram_s16x8* frag = reinterpret_cast<gram_s16x8*>(lds);  // [cb][q][64 lanes]
+  __shared__ uint32_t sh_off;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int grp = lane >> 4;
+  int64_t base;
+  int n;
+  int wid;
+  win.get(blockIdx.x, wid, base, n);
+  const float* xb = x + (size_t)base * d;
+  const int ncb = (n + 15) >> 4;
+  uint32_t* rowcnt = reinterpret_cast<uint32_t*>(lds + (size_t)ncb * 2048);  // [n]
+  if (tid == 0) sh_off = 0;
+  if (tid < n) rowcnt[tid] = 0;
+  __syncthreads();
+
+  // Column operands: thread j writes column j's eight 16-byte entries
+  // [hi(-2x) | lo(-2x) | hi(-2x) | norm slots | zeros].  One column per
+  // thread: blockDim.x >= 64 ceil(n / 64) >= 16 ncb.
+  int bad = 0;
+  float sqj = 0.f;
+  const int j = tid;
+  const bool colw = j < 16 * ncb;
+  if (colw) {
+    gram_s16x8 bh[2] = {}, bl[2] = {};
+    if (j < n) {
+      float v[DK];
+      sqj = window_point(xb + (size_t)j * d, d, v, bad);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float w = -2.f * v[8 * h + e];
+          const uint32_t g = bf16_bits(w);
+          bh[h][e] = (short)g;
+          bl[h][e] = (short)bf16_bits(w - bf16_value(g));
+        }
+      // NaN compares false: a non-finite norm turns the filter off too.
+      if (!(sqj <= 1.2676506002282294e30f)) atomicOr(&sh_off, 1u);
+    }
+    const int cl = j & 15;
+    gram_s16x8* f0 = frag + (size_t)(j >> 4) * 128 + cl;
+    f0[0] = bh[0];
+    f0[16] = bh[1];
+    f0[32] = bl[0];
+    f0[48] = bl[1];
+    f0[64] = bh[0];
+    f0[80] = bh[1];
+    f0[96] = norm_slots(fmaf(sqj, kWinC, sqj) + kGramEh, false, j >= n);
+    f0[112] = gram_s16x8{};
+  }
+  if (__any(bad) && lane == 0) atomicOr(status, 2);
+  __syncthreads();
+  const int row0 = wave * 16 * RS;
+  const bool active = row0 < n;
+  if (sh_off) {
+    // Filter off for this window: every row goes to the exact path.
+    if (tid < n) ncand[base + tid] = CAP + 1;
+    return;
+  }
+
+  // Row operands of set g: row row0 + 16 g + (lane & 15) as
+  // [hi(x) | hi(x) | lo(x) | norm slots | zeros] (lane holds K entries
+  // 8 grp .. 8 grp + 7 of each 32-wide MFMA step).
+  gram_bf16x8 av[RS][2];
+  float sqr[RS];
+  if (active) {
+#pragma unroll
+    for (int g = 0; g < RS; ++g) {
+      const int i = min(row0 + 16 * g + (lane & 15), n - 1);
+      float v[DK];
+      int ignore = 0;
+      sqr[g] = window_point(xb + (size_t)i * d, d, v, ignore);
+      const int h = grp & 1;
+      gram_s16x8 hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float c = v[8 * h + e];
+        const uint32_t b = bf16_bits(c);
+        hi[e] = (short)b;
+        lo[e] = (short)bf16_bits(c - bf16_value(b));
+      }
+      av[g][0] = __builtin_bit_cast(gram_bf16x8, hi);
+      const gram_s16x8 q1 = grp < 2 ? lo : grp == 2 ? norm_slots(fmaf(sqr[g], kWinC, sqr[g]) + kGramEh, true, false)
+                                                    : gram_s16x8{};
+      av[g][1] = __builtin_bit_cast(gram_bf16x8, q1);
+    }
+  }
+
+  // Bounds of one column block for the four sets: all eight MFMAs issue
+  // before any result is read.
+  auto block = [&](const gram_bf16x8& b0, const gram_bf16x8& b1, gram_f32x4 (&acc)[RS]) {
+#pragma unroll
+    for (int g = 0; g < RS; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, av[g][0], gram_f32x4{}, 0, 0, 0);
+#pragma unroll
+    for (int g = 0; g < RS; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, av[g][1], acc[g], 0, 0, 0);
+  };
+  auto load = [&](int cb, gram_bf16x8& b0, gram_bf16x8& b1) {
+    b0 = __builtin_bit_cast(gram_bf16x8, frag[cb * 128 + lane]);
+    b1 = __builtin_bit_cast(gram_bf16x8, frag[cb * 128 + 64 + lane]);
+  };
+
+  float U[RS];
+  if (active) {
+    // Pass A.
+    float mn[RS][M];
+#pragma unroll
+    for (int g = 0; g < RS; ++g)
+#pragma unroll
+      for (int q = 0; q < M; ++q) mn[g][q] = INFINITY;
+    gram_bf16x8 b0, b1;
+    load(0, b0, b1);
+    for (int cb = 0; cb < ncb; ++cb) {
+      gram_f32x4 acc[RS];
+      block(b0, b1, acc);
+      load(min(cb + 1, ncb - 1), b0, b1);
+#pragma unroll
+      for (int g = 0; g < RS; ++g)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          // Ascending list of the M smallest: the new q-th smallest is the
+          // median of (lst[q-1], lst[q], v); the smallest an unsigned
+          // minimum of the bit patterns (positive bounds or +inf order like
+          // their bits; no NaN: the filter runs on finite norms only).
+          const float v = acc[g][r];
+#pragma unroll
+          for (int q = M - 1; q > 0; --q) mn[g][q] = __builtin_amdgcn_fmed3f(mn[g][q - 1], mn[g][q], v);
+          mn[g][0] = __uint_as_float(min(__float_as_uint(v), __float_as_uint(mn[g][0])));
+        }
+    }
+    // U: the ms-th smallest of the row's 4 M kept values (lanes l & 15 +
+    // 16 k), by ms extractions of the minimum, one instance at a time.
+#pragma unroll
+    for (int g = 0; g < RS; ++g) {
+      float all[4 * M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        all[q] = mn[g][q];
+        all[M + q] = __shfl_xor(mn[g][q], 16);
+        all[2 * M + q] = __shfl_xor(mn[g][q], 32);
+        all[3 * M + q] = __shfl_xor(mn[g][q], 48);
+      }
+      float u = INFINITY;
+      for (int it = 0; it < ms; ++it) {
+        float m = all[0];
+#pragma unroll
+        for (int q = 1; q < 4 * M; ++q) m = fminf(m, all[q]);
+        bool done = false;
+#pragma unroll
+        for (int q = 0; q < 4 * M; ++q) {
+          const bool hit = !done && all[q] == m;
+          all[q] = hit ? INFINITY : all[q];
+          done = done || hit;
+        }
+        u = m;
+      }
+      U[g] = u;
+    }
+  }
+  // Pass B operands: the lower-bound norm terms replace the upper ones.
+  __syncthreads();  // every wave has finished reading pass A's fragments
+  if (colw) frag[(size_t)(j >> 4) * 128 + 96 + (j & 15)] = norm_slots(fmaf(sqj, -kWinC, sqj) - kGramEh, false, j >= n);
+  if (active && grp == 2) {
+#pragma unroll
+    for (int g = 0; g < RS; ++g)
+      av[g][1] = __builtin_bit_cast(gram_bf16x8, norm_slots(fmaf(sqr[g], -kWinC, sqr[g]) - kGramEh, true, false));
+  }
+  __syncthreads();
+  if (!active) return;
+
+  // Pass B.
+  gram_bf16x8 b0, b1;
+  load(0, b0, b1);
+  for (int c8 = 0; c8 < ncb; c8 += 8) {
+    uint32_t word[RS] = {};
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) {
+      const int cb = c8 + cc;
+      if (cb < ncb) {
+        gram_f32x4 acc[RS];
+        block(b0, b1, acc);
+        load(min(cb + 1, ncb - 1), b0, b1);
+#pragma unroll
+        for (int g = 0; g < RS; ++g)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) word[g] |= (acc[g][r] > U[g] ? 0u : 1u) << (4 * cc + r);
+      }
+    }
+    // Expand the (rare) set bits: column 16 (c8 + b / 4) + 4 grp + b % 4 of
+    // row row0 + 16 g + (lane & 15); rows past the window are never listed.
+#pragma unroll
+    for (int g = 0; g < RS; ++g) {
+      const int i = row0 + 16 * g + (lane & 15);
+      uint32_t m = i < n ? word[g] : 0u;
+      while (m) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1u;
+        const int jc = 16 * (c8 + (b >> 2)) + 4 * grp + (b & 3);
+        if (jc < n) {
+          const uint32_t pos = atomicAdd(&rowcnt[i], 1u);
+          if (pos < (uint32_t)CAP) cand[(base + i) * CAP + pos] = (uint16_t)jc;
+        }
+      }
+    }
+  }
+  // A row's four lanes are in this wave and their counter updates have
+  // completed in program order.
+#pragma unroll
+  for (int g = 0; g < RS; ++g) {
+    const int i = row0 + 16 * g + (lane & 15);
+    if (grp == 0 && i < n) ncand[base + i] = (int32_t)rowcnt[i];
+  }
+}
+
 // The same outputs for DK <= 16 with a block's 256 points staged through
 // LDS: the rows are read and the bf16 pieces written as contiguous 16-byte
 // vectors across lanes (one point per lane would touch 64 cache lines per

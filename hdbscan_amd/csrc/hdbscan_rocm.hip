@@ -118,6 +118,7 @@ struct Plan {
   bool core_topk = false; // core from points with a register list
   int kmax = 0, core_dk = 0;
   bool core_gram = false; // core through the certified MFMA Gram filter
+  bool gram_window = false;  // ... in its per-window form (core_window_kernel)
   int gram_dk = 0, gram_rs = 0, gram_threads = 0;
   uint32_t gram_lds = 0;
   bool tree_lds = true;
@@ -249,6 +250,10 @@ void set_gram(Plan& P, int64_t n, int64_t d, int32_t ms, uint32_t lds_limit) {
   while (waves > 1 && (int64_t)(waves / 2) * wr >= n) waves /= 2;
   if (gram_bytes(dk, rs, n) > lds_limit - 1024u) return;
   P.core_gram = true;
+  // Windows of up to 1024 points and 16 dimensions: one workgroup per window
+  // with the column operands in LDS once, no prep kernel.
+  P.gram_window = dk <= kWinDK && n <= kWinMaxN && !P.matrix && !(P.flags & HDB_FLAG_GRAM_CHUNKED) &&
+                  gram_window_lds_bytes((int)n) <= lds_limit;
   P.gram_dk = dk;
   P.gram_rs = rs;
   P.gram_threads = waves * 64;
@@ -351,7 +356,7 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   P.tree_lds_bytes = P.tree_lds ? L.total : 0u;
   P.arena_stride = P.tree_lds ? 0 : align256(L.total);
   P.need_matrix = !(flags & HDB_FLAG_MATRIX_INPUT) && (P.matrix || (P.core && !P.core_topk));
-  const bool pieces_needed = P.core_gram || P.gram_graph;
+  const bool pieces_needed = (P.core_gram && !P.gram_window) || P.gram_graph;
   size_t o = 0;
   const size_t bn = (size_t)batch * n;
   P.off_eu = o; o = align256(o + bn * 4);
@@ -586,14 +591,24 @@ int run_gram_graph(hipStream_t s, const Plan& P, const float* x, const Launch& L
 // unless three of the nearest share a lane).
 int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist, const Launch& L, int64_t d,
                 int ms, float* core, unsigned char* ws, int32_t* status, bool prep) {
-  if (prep) {
-    if (int r = launch_gram_prep(s, P, x, L.points, d, ws, status)) return r;
-    stage_mark(s, "gram_prep_kernel");
-  }
   const GramBuffers g = gram_buffers(P, ws);
-  const int r = ms <= 3 ? launch_gram_d<1>(s, P, L, ms, g) : launch_gram_d<2>(s, P, L, ms, g);
-  if (r) return r;
-  stage_mark(s, "core_gram_kernel");
+  if (P.gram_window && !dist) {
+    const int threads = (int)std::min<int64_t>(1024, (L.nmax + 63) / 64 * 64);
+    const uint32_t lds = gram_window_lds_bytes((int)L.nmax);
+    auto fn = ms <= 8 ? core_window_kernel<2> : core_window_kernel<4>;
+    if (int r = allow_lds((const void*)fn, lds)) return r;
+    fn<<<(unsigned)L.count, threads, lds, s>>>(x, L.win, (int)d, ms, g.cand, g.ncand, status);
+    if (int r = check_launch("core_window_kernel")) return r;
+    stage_mark(s, "core_window_kernel");
+  } else {
+    if (prep) {
+      if (int r = launch_gram_prep(s, P, x, L.points, d, ws, status)) return r;
+      stage_mark(s, "gram_prep_kernel");
+    }
+    const int r = ms <= 3 ? launch_gram_d<1>(s, P, L, ms, g) : launch_gram_d<2>(s, P, L, ms, g);
+    if (r) return r;
+    stage_mark(s, "core_gram_kernel");
+  }
   const uint64_t xbytes = (uint64_t)L.nmax * d * sizeof(float);
   if (!dist && xbytes <= kExactLdsBytes) {
     // The window's points fit in LDS: one workgroup per window reads them
@@ -958,7 +973,8 @@ struct RaggedPlan {
   int64_t count[kBuckets] = {}, nmax[kBuckets] = {}, start[kBuckets + 1] = {};
   Plan plan[kBuckets];
   bool used[kBuckets] = {};
-  bool gram = false, global_arena = false;
+  bool gram = false, global_arena = false;  // gram: some bucket reads the prep kernel's pieces
+  bool gram_cand = false;                   // some bucket writes Gram candidate lists
   int gram_bucket = -1;
   int top = -1;       // bucket of the largest windows
   int64_t head = 0;   // its leading windows run as a separate pass (0: one pass)
@@ -1033,14 +1049,16 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
     if (make_plan(R.count[k], R.nmax[k], d, dtype, ms, fk, &R.plan[k])) return -1;
     const Plan& P = R.plan[k];
     if (P.matrix) R.matrix_mask |= 1u << k;
-    if (P.core_gram) {
+    if (P.core_gram) R.gram_cand = true;
+    if (P.core_gram && !P.gram_window) {  // the chunked form reads the prep kernel's pieces
       R.gram = true;
       R.gram_bucket = k;
     }
     if (!P.tree_lds) R.global_arena = true;
   }
   for (int k = 0; k < kBuckets; ++k)
-    if (R.used[k] && R.plan[k].core_gram && R.plan[k].gram_dk != R.plan[R.gram_bucket].gram_dk)
+    if (R.used[k] && R.plan[k].core_gram && !R.plan[k].gram_window &&
+        R.plan[k].gram_dk != R.plan[R.gram_bucket].gram_dk)
       return fail("internal: Gram paddings differ between buckets");
   for (int64_t w = 0; w < windows; ++w) {
     const int64_t n = off[w + 1] - off[w];
@@ -1058,8 +1076,8 @@ int make_ragged_plan(const int64_t* off, int64_t windows, int64_t d, int dtype, 
   R.off_arena = o; o = align256(o + (R.global_arena ? pts * kArenaPerPoint + (size_t)windows * kArenaPerWindow : 0));
   R.off_slots = o;
   o = align256(o + pts * (rs == 4 ? sizeof(ClusterSlot<float>) : sizeof(ClusterSlot<double>)));
-  R.off_cand = o; o = align256(o + (R.gram ? pts * kGramCap * 2 : 0));
-  R.off_ncand = o; o = align256(o + (R.gram ? pts * 4 : 0));
+  R.off_cand = o; o = align256(o + (R.gram_cand ? pts * kGramCap * 2 : 0));
+  R.off_ncand = o; o = align256(o + (R.gram_cand ? pts * 4 : 0));
   const size_t pieces = R.gram ? pts * 2 * (size_t)gram_dk * 2 : 0;
   R.off_pa = o; o = align256(o + pieces);
   R.off_pb = o; o = align256(o + pieces);
@@ -1485,11 +1503,17 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
   // The cluster plan with a materialised matrix lays out the matrix (for
   // min_samples > 64) and the Gram candidate lists (3 <= min_samples <= 16).
   Plan P;
+  // HDB_FLAG_NO_GRAM_EXACT: the MFMA-bounded matrix selects core distances
+  // inside its own stage; this entry runs the core stage alone.
   const int filter = (flags & HDB_FLAG_PLAIN) ? 0 : HDB_FLAG_FILTER;
-  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX | filter, &P)) return -1;
+  if (make_plan(batch, n, d, dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_NO_GRAM_EXACT | filter, &P))
+    return -1;
   P.core = true;
   P.core_topk = min_samples <= 64;
   P.kmax = min_samples <= 8 ? 8 : min_samples <= 16 ? 16 : min_samples <= 32 ? 32 : 64;
+  // No matrix is materialised before the core stage here, so the Gram filter
+  // takes its per-window form where the clustering path would.
+  P.gram_window = P.core_gram && P.gram_dk <= kWinDK && n <= kWinMaxN && !(flags & HDB_FLAG_GRAM_CHUNKED);
   const size_t bn = (size_t)batch * n;
   const size_t need = P.core_gram ? P.total : P.core_topk ? 0 : P.off_matrix + bn * n * P.rs;
   if (workspace_bytes < need || (need && !workspace)) return fail("workspace too small");

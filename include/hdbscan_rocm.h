@@ -52,6 +52,7 @@ extern "C" {
 #define HDB_FLAG_PLAIN 64       /* no MFMA Gram filter: every core distance exact */
 #define HDB_FLAG_FILTER 128     /* MFMA Gram filter also on the matrix path */
 #define HDB_FLAG_ONE_STEP 65536 /* matrix-path Prim: one vertex per step, no row speculation */
+#define HDB_FLAG_GRAM_CHUNKED 131072 /* core candidates by the chunked Gram kernel also for windows of <= 1024 points, <= 16 dims */
 #define HDB_FLAG_GRAM_GRAPH 262144 /* float32 matrix path: MFMA Gram matrix + interval-key Prim (no exact matrix) */
 #define HDB_FLAG_NO_GRAPH 524288    /* ragged calls: launch directly instead of capturing / replaying a HIP graph */
 #define HDB_FLAG_SPLIT_TREE 1048576 /* tree stage as a sort kernel + a narrow hierarchy kernel (sorted-edge transport) */
@@ -158,9 +159,12 @@ int hdb_bounded_distances(void *stream, const void *x, int64_t batch, int64_t n,
 
 /* Core distances core[batch][n]: the (min_samples-1)-th smallest distance of
  * each row, self included.  Workspace: hdb_workspace_bytes(batch, n, d,
- * dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_FILTER) bytes
- * always suffice (none is used when min_samples <= 64 and the MFMA filter
- * does not apply).  The filter is used unless flags has HDB_FLAG_PLAIN.
+ * dtype, min_samples, flags | HDB_FLAG_MATRIX | HDB_FLAG_FILTER |
+ * HDB_FLAG_NO_GRAM_EXACT) bytes always suffice (none is used when
+ * min_samples <= 64 and the MFMA filter does not apply).  The filter is used
+ * unless flags has HDB_FLAG_PLAIN: its per-window form for windows of up to
+ * 1024 points and 16 dimensions (unless HDB_FLAG_GRAM_CHUNKED), the chunked
+ * form otherwise.
  * Replaces reference hdbscan_gpu/_cuda.py:114-118 (cp.partition). */
 int hdb_core_distances(void *stream, const void *x, int64_t batch, int64_t n,
                        int64_t d, int dtype, int32_t min_samples, void *core,

@@ -96,7 +96,8 @@ def core(x, min_samples, precision, flags=0):
     out = t.empty((b, n), dtype=xd.dtype, device="cuda")
     status = t.zeros(1, dtype=t.int32, device="cuda")
     ws_bytes = _native.lib().hdb_workspace_bytes(b, n, d, code(precision), int(min_samples),
-                                                 int(flags) | 2 | (0 if flags & 64 else 128))
+                                                 int(flags) | 2 | _native.FLAG_NO_GRAM_EXACT
+                                                 | (0 if flags & 64 else 128))
     assert ws_bytes > 0, _native.lib().hdb_last_error()
     ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
     _native.check(_native.lib().hdb_core_distances(stream(), ptr(xd), b, n, d, code(precision), int(min_samples),

@@ -44,10 +44,13 @@ def test_version_and_workspace_sizing():
     assert lib.hdb_version() >= 100
     ws = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, _native.FLAG_PLAIN)
     assert ws > 256 * 1024 * 12
-    # The MFMA filter adds per point: bf16 pieces (2 x 2 x 16 x 2 B), norm
-    # terms (12 B), candidate lists (32 x 2 B) and counts (4 B).
+    # The MFMA filter adds per point: candidate lists (32 x 2 B) and counts
+    # (4 B); its chunked form also bf16 pieces (2 x 2 x 16 x 2 B) and norm
+    # terms (12 B), which the per-window form computes in LDS.
     wf = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, 0)
-    assert wf >= ws + 256 * 1024 * (128 + 12 + 64 + 4)
+    assert wf >= ws + 256 * 1024 * (64 + 4)
+    wc = lib.hdb_workspace_bytes(256, 1024, 16, _native.F32, 5, _native.FLAG_GRAM_CHUNKED)
+    assert wc >= ws + 256 * 1024 * (128 + 12 + 64 + 4)
     # The points path keeps 16 B per point for Prim's repacking.
     assert ws >= 256 * 1024 * (12 + 16)
     # The materialised matrix path needs B*N*N reals (and no repacking).
@@ -99,8 +102,8 @@ def test_product_path_never_references_the_oracle():
 def test_noalloc_rejects_calls_above_its_scratch_before_any_device_work():
     lib = _native.lib()
     p = ctypes.c_void_p(1)
-    assert lib.hdb_workspace_bytes(512, 1024, 16, _native.F32, 5, 0) > _native.NOALLOC_BYTES
-    rc = lib.hdb_cluster_noalloc(None, p, 512, 1024, 16, _native.F32, 5, 5, 0.0, p, p, 0)
+    assert lib.hdb_workspace_bytes(1024, 1024, 16, _native.F32, 5, 0) > _native.NOALLOC_BYTES
+    rc = lib.hdb_cluster_noalloc(None, p, 1024, 1024, 16, _native.F32, 5, 5, 0.0, p, p, 0)
     assert rc != 0 and "scratch" in lib.hdb_last_error().decode()
     rc = lib.hdb_cluster_noalloc(None, None, 1, 8, 2, _native.F32, 2, 2, 0.0, p, p, 0)
     assert rc != 0 and "null" in lib.hdb_last_error().decode()

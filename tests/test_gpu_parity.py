@@ -7,7 +7,7 @@ import pytest
 import gpu_helpers as G
 from golden_io import records, same_partition, workload_windows
 from oracle import oracle
-from hdbscan_amd._native import FLAG_GRAM_GRAPH, FLAG_ONE_STEP, FLAG_SERIAL_LINKAGE, FLAG_SPLIT_TREE
+from hdbscan_amd._native import FLAG_GRAM_CHUNKED, FLAG_GRAM_GRAPH, FLAG_ONE_STEP, FLAG_SERIAL_LINKAGE, FLAG_SPLIT_TREE
 
 pytestmark = pytest.mark.gpu
 
@@ -88,14 +88,16 @@ def gram_core_cached():
 
 
 @pytest.mark.parametrize("ms", [3, 4, 8, 9, 16])
-@pytest.mark.parametrize("rsets", [0, 1, 2, 3])
-def test_core_gram_filter_bitwise(ms, rsets):
-    """rsets: flag bits 12..13, 16-row sets per wave of the Gram kernel (0:
-    by window size, four for 1025..2048 points; 1, 2, 3: one, two, four)."""
+@pytest.mark.parametrize("rsets,chunked", [(0, 0)] + [(r, FLAG_GRAM_CHUNKED) for r in range(4)])
+def test_core_gram_filter_bitwise(ms, rsets, chunked):
+    """rsets: flag bits 12..13, 16-row sets per wave of the chunked Gram
+    kernel (0: by window size, four for 1025..2048 points; 1, 2, 3: one, two,
+    four).  chunked 0: windows of up to 1024 points and 16 dimensions take the
+    per-window kernel (column operands in LDS, norms in the MFMA's K slots)."""
     for name, x in gram_core_cached().items():
         if ms > len(x):
             continue
-        got, status = G.core(x, ms, "float32", rsets << 12)
+        got, status = G.core(x, ms, "float32", (rsets << 12) | chunked)
         assert status == 0, name
         exp = oracle.core(oracle.distances(x, "float32"), ms, "float32")
         np.testing.assert_array_equal(got[0].view(np.uint32), exp.view(np.uint32), err_msg=name)

@@ -67,7 +67,7 @@ def test_noalloc_status_and_limit(torch):
     assert status == 0
     lib = _native.lib()
     p = ctypes.c_void_p(1)
-    # 512 windows of 1024 x 16 float32 need more than the fixed scratch.
-    assert lib.hdb_workspace_bytes(512, 1024, 16, 0, 5, 0) > _native.NOALLOC_BYTES
-    rc = lib.hdb_cluster_noalloc(None, p, 512, 1024, 16, 0, 5, 5, 0.0, p, p, 0)
+    # 1024 windows of 1024 x 16 float32 need more than the fixed scratch.
+    assert lib.hdb_workspace_bytes(1024, 1024, 16, 0, 5, 0) > _native.NOALLOC_BYTES
+    rc = lib.hdb_cluster_noalloc(None, p, 1024, 1024, 16, 0, 5, 5, 0.0, p, p, 0)
     assert rc != 0 and "scratch" in lib.hdb_last_error().decode()
