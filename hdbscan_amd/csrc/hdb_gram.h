@@ -469,14 +469,30 @@ __device__ __forceinline__ gram_s16x8 norm_slots(float v, bool row_side, bool in
   return s;
 }
 
-// Loads point p's DK padded coordinates (zeros past d), returns its squared
-// norm (FMA chain in coordinate order, as gram_prep_kernel) and ORs 1 into
-// bad for a non-finite coordinate.
-__device__ __forceinline__ float window_point(const float* xp, int d, float (&v)[kWinDK], int& bad) {
+// Loads a point's DK padded coordinates (zeros past d): four 16-byte loads
+// when d == DK and the row is 16-byte aligned.
+__device__ __forceinline__ void window_load(const float* xp, int d, float (&v)[kWinDK]) {
+  if (d == kWinDK && (reinterpret_cast<uintptr_t>(xp) & 15) == 0) {
+#pragma unroll
+    for (int q = 0; q < kWinDK / 4; ++q) {
+      const float4 t = reinterpret_cast<const float4*>(xp)[q];
+      v[4 * q] = t.x;
+      v[4 * q + 1] = t.y;
+      v[4 * q + 2] = t.z;
+      v[4 * q + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kWinDK; ++k) v[k] = k < d ? xp[k] : 0.f;
+  }
+}
+
+// Squared norm of a loaded point (FMA chain in coordinate order, as
+// gram_prep_kernel); ORs 1 into bad for a non-finite coordinate.
+__device__ __forceinline__ float window_norm(const float (&v)[kWinDK], int& bad) {
   float sq = 0.f;
 #pragma unroll
   for (int k = 0; k < kWinDK; ++k) {
-    v[k] = k < d ? xp[k] : 0.f;
     bad |= !isfinite(v[k]);
     sq = fmaf(v[k], v[k], sq);
   }
@@ -537,7 +553,8 @@ __global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windo
     gram_s16x8 bh[2] = {}, bl[2] = {};
     if (j < n) {
       float v[DK];
-      sqj = window_point(xb + (size_t)j * d, d, v, bad);
+      window_load(xb + (size_t)j * d, d, v);
+      sqj = window_norm(v, bad);
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -577,12 +594,15 @@ __global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windo
   gram_bf16x8 av[RS][2];
   float sqr[RS];
   if (active) {
+    // All four rows' loads are issued before any arithmetic.
+    float vr[RS][DK];
+#pragma unroll
+    for (int g = 0; g < RS; ++g) window_load(xb + (size_t)min(row0 + 16 * g + (lane & 15), n - 1) * d, d, vr[g]);
 #pragma unroll
     for (int g = 0; g < RS; ++g) {
-      const int i = min(row0 + 16 * g + (lane & 15), n - 1);
-      float v[DK];
+      const float (&v)[DK] = vr[g];
       int ignore = 0;
-      sqr[g] = window_point(xb + (size_t)i * d, d, v, ignore);
+      sqr[g] = window_norm(v, ignore);
       const int h = grp & 1;
       gram_s16x8 hi, lo;
 #pragma unroll
@@ -623,9 +643,16 @@ __global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windo
     gram_bf16x8 b0, b1;
     load(0, b0, b1);
     for (int cb = 0; cb < ncb; ++cb) {
+      // The next block's fragments are read before this block's MFMAs (the
+      // scheduling barrier keeps the compiler from sinking the reads to
+      // their use), so the LDS latency overlaps this block's work.
+      gram_bf16x8 n0, n1;
+      load(min(cb + 1, ncb - 1), n0, n1);
+      __builtin_amdgcn_sched_barrier(0);
       gram_f32x4 acc[RS];
       block(b0, b1, acc);
-      load(min(cb + 1, ncb - 1), b0, b1);
+      b0 = n0;
+      b1 = n1;
 #pragma unroll
       for (int g = 0; g < RS; ++g)
 #pragma unroll
@@ -689,9 +716,13 @@ __global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windo
     for (int cc = 0; cc < 8; ++cc) {
       const int cb = c8 + cc;
       if (cb < ncb) {
+        gram_bf16x8 n0, n1;
+        load(min(cb + 1, ncb - 1), n0, n1);
+        __builtin_amdgcn_sched_barrier(0);
         gram_f32x4 acc[RS];
         block(b0, b1, acc);
-        load(min(cb + 1, ncb - 1), b0, b1);
+        b0 = n0;
+        b1 = n1;
 #pragma unroll
         for (int g = 0; g < RS; ++g)
 #pragma unroll
