@@ -172,7 +172,7 @@ def roofline_for(dominant, ms, batch, n, d, precision, valu_tops=None, matrix=Fa
     itemsize = 4 if precision == "float32" else 8
     t = ms * 1e-3
     pairs = n * (n - 1) / 2
-    if dominant == "core_gram_kernel":
+    if dominant in ("core_gram_kernel", "core_window_kernel"):
         flops = 2.0 * d * n * n * batch
         return roofline_entry("mfma", flops / t / 1e12, BF16_PEAK_TFLOPS, "TFLOP/s", dominant,
                               f"{flops:.4g} FLOP per launch (Gram 2*D*N^2 per window)")
@@ -728,7 +728,7 @@ def main(argv=None):
         "config": {"workload": args.workload, "batch": batch, "n": n, "d": d,
                    "min_cluster_size": mcs, "min_samples": ms, "cluster_selection_epsilon": eps,
                    "path": ("MFMA-bounded matrix" if "gx_matrix_kernel" in stages else
-                            "MFMA Gram filter" if any(k.startswith("gram") or k.startswith("core_gram")
+                            "MFMA Gram filter" if any(k.startswith(("gram", "core_gram", "core_window"))
                                                       for k in stages) else "plain exact-distance"),
                    "flags": args.flags},
         "label_exact_windows": exact,

@@ -434,9 +434,16 @@ constexpr int kWinMaxN = 1024;
 // Norm-term constant of the per-window form.
 constexpr float kWinC = gram_c(kWinDK + 4);
 
-// LDS bytes of core_window_kernel for windows of up to n points: two 1 KiB
-// fragments per 16-column block, then a candidate counter per row.
-HDB_HD inline uint32_t gram_window_lds_bytes(int n) { return (uint32_t)((n + 15) / 16) * 2048u + 4u * (uint32_t)n; }
+// LDS bytes of core_window_kernel for windows of up to n points of d
+// dimensions: two counters per row, then either the column fragments (two
+// 1 KiB fragments per 16-column block) or, after the passes, the window's
+// points and the rows' candidate lists.
+HDB_HD inline uint32_t gram_window_head(int n) { return (8u * (uint32_t)n + 15u) & ~15u; }
+HDB_HD inline uint32_t gram_window_lds_bytes(int n, int d) {
+  const uint32_t frags = (uint32_t)((n + 15) / 16) * 2048u;
+  const uint32_t exact = ((4u * (uint32_t)n * (uint32_t)d + 15u) & ~15u) + 2u * kGramCap * (uint32_t)n;
+  return gram_window_head(n) + (frags > exact ? frags : exact);
+}
 
 // The three bf16 pieces of v (finite) as bit patterns.
 __device__ __forceinline__ void bf16_pieces(float v, uint32_t& p1, uint32_t& p2, uint32_t& p3) {
@@ -497,262 +504,6 @@ __device__ __forceinline__ float window_norm(const float (&v)[kWinDK], int& bad)
     sq = fmaf(v[k], v[k], sq);
   }
   return sq;
-}
-
-// Outputs as core_gram_kernel: cand[point][kGramCap] column indices and
-// ncand[point] (> kGramCap: recompute the row exactly; the whole window
-// when some squared norm exceeds 2^100 or is not finite).  Grid: one
-// workgroup per window of the launch, blockDim.x >= the largest window
-// rounded up to 64, LDS gram_window_lds_bytes(largest window).
-//
-// The column fragment is the MFMA's first operand and the row fragment the
-// second, so accumulator register r of lane l holds column 4 (l >> 4) + r of
-// the block and row l & 15 of the set: each lane owns one row per set and
-// sees four of its columns per block (the row's 16 columns of a block are
-// spread over the lanes l & 15 + 16 k, k = 0..3).
-//   Pass A: each lane keeps the M smallest upper bounds of its columns; U =
-//           the ms-th smallest of the 4 M values of the row's four lanes (a
-//           subset of the row, so U >= the ms-th smallest upper bound >= the
-//           exact statistic).  M = 2 for ms <= 8, 4 for ms <= 16: 4 M >= ms.
-//   Pass B: a column is a candidate when its lower bound is <= U.  Each
-//           lane ORs the test results of eight blocks into one mask word
-//           (bit 4 c + r: block c of the eight, register r) and only set
-//           bits are expanded: a slot from the row's LDS counter (atomic
-//           add), the column written to the row's list when the slot is
-//           below kGramCap.  The counter is the row's candidate count.
-// M: values kept per lane in pass A (2, 4).
-template <int M>
-__global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windows win, int d, int ms,
-                                                           uint16_t* cand, int32_t* ncand, int32_t* status) {
-  constexpr int DK = kWinDK, CAP = kGramCap, RS = 4;
-  extern __shared__ __align__(16) unsigned char lds[];
-  gram_s16x8* frag = reinterpret_cast<gram_s16x8*>(lds);  // [cb][q][64 lanes]
-  __shared__ uint32_t sh_off;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int grp = lane >> 4;
-  int64_t base;
-  int n;
-  int wid;
-  win.get(blockIdx.x, wid, base, n);
-  const float* xb = x + (size_t)base * d;
-  const int ncb = (n + 15) >> 4;
-  uint32_t* rowcnt = reinterpret_cast<uint32_t*>(lds + (size_t)ncb * 2048);  // [n]
-  if (tid == 0) sh_off = 0;
-  if (tid < n) rowcnt[tid] = 0;
-  __syncthreads();
-
-  // Column operands: thread j writes column j's eight 16-byte entries
-  // [hi(-2x) | lo(-2x) | hi(-2x) | norm slots | zeros].  One column per
-  // thread: blockDim.x >= 64 ceil(n / 64) >= 16 ncb.
-  int bad = 0;
-  float sqj = 0.f;
-  const int j = tid;
-  const bool colw = j < 16 * ncb;
-  if (colw) {
-    gram_s16x8 bh[2] = {}, bl[2] = {};
-    if (j < n) {
-      float v[DK];
-      window_load(xb + (size_t)j * d, d, v);
-      sqj = window_norm(v, bad);
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const float w = -2.f * v[8 * h + e];
-          const uint32_t g = bf16_bits(w);
-          bh[h][e] = (short)g;
-          bl[h][e] = (short)bf16_bits(w - bf16_value(g));
-        }
-      // NaN compares false: a non-finite norm turns the filter off too.
-      if (!(sqj <= 1.2676506002282294e30f)) atomicOr(&sh_off, 1u);
-    }
-    const int cl = j & 15;
-    gram_s16x8* f0 = frag + (size_t)(j >> 4) * 128 + cl;
-    f0[0] = bh[0];
-    f0[16] = bh[1];
-    f0[32] = bl[0];
-    f0[48] = bl[1];
-    f0[64] = bh[0];
-    f0[80] = bh[1];
-    f0[96] = norm_slots(fmaf(sqj, kWinC, sqj) + kGramEh, false, j >= n);
-    f0[112] = gram_s16x8{};
-  }
-  if (__any(bad) && lane == 0) atomicOr(status, 2);
-  __syncthreads();
-  const int row0 = wave * 16 * RS;
-  const bool active = row0 < n;
-  if (sh_off) {
-    // Filter off for this window: every row goes to the exact path.
-    if (tid < n) ncand[base + tid] = CAP + 1;
-    return;
-  }
-
-  // Row operands of set g: row row0 + 16 g + (lane & 15) as
-  // [hi(x) | hi(x) | lo(x) | norm slots | zeros] (lane holds K entries
-  // 8 grp .. 8 grp + 7 of each 32-wide MFMA step).
-  gram_bf16x8 av[RS][2];
-  float sqr[RS];
-  if (active) {
-    // All four rows' loads are issued before any arithmetic.
-    float vr[RS][DK];
-#pragma unroll
-    for (int g = 0; g < RS; ++g) window_load(xb + (size_t)min(row0 + 16 * g + (lane & 15), n - 1) * d, d, vr[g]);
-#pragma unroll
-    for (int g = 0; g < RS; ++g) {
-      const float (&v)[DK] = vr[g];
-      int ignore = 0;
-      sqr[g] = window_norm(v, ignore);
-      const int h = grp & 1;
-      gram_s16x8 hi, lo;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const float c = v[8 * h + e];
-        const uint32_t b = bf16_bits(c);
-        hi[e] = (short)b;
-        lo[e] = (short)bf16_bits(c - bf16_value(b));
-      }
-      av[g][0] = __builtin_bit_cast(gram_bf16x8, hi);
-      const gram_s16x8 q1 = grp < 2 ? lo : grp == 2 ? norm_slots(fmaf(sqr[g], kWinC, sqr[g]) + kGramEh, true, false)
-                                                    : gram_s16x8{};
-      av[g][1] = __builtin_bit_cast(gram_bf16x8, q1);
-    }
-  }
-
-  // Bounds of one column block for the four sets: all eight MFMAs issue
-  // before any result is read.
-  auto block = [&](const gram_bf16x8& b0, const gram_bf16x8& b1, gram_f32x4 (&acc)[RS]) {
-#pragma unroll
-    for (int g = 0; g < RS; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, av[g][0], gram_f32x4{}, 0, 0, 0);
-#pragma unroll
-    for (int g = 0; g < RS; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, av[g][1], acc[g], 0, 0, 0);
-  };
-  auto load = [&](int cb, gram_bf16x8& b0, gram_bf16x8& b1) {
-    b0 = __builtin_bit_cast(gram_bf16x8, frag[cb * 128 + lane]);
-    b1 = __builtin_bit_cast(gram_bf16x8, frag[cb * 128 + 64 + lane]);
-  };
-
-  float U[RS];
-  if (active) {
-    // Pass A.
-    float mn[RS][M];
-#pragma unroll
-    for (int g = 0; g < RS; ++g)
-#pragma unroll
-      for (int q = 0; q < M; ++q) mn[g][q] = INFINITY;
-    gram_bf16x8 b0, b1;
-    load(0, b0, b1);
-    for (int cb = 0; cb < ncb; ++cb) {
-      // The next block's fragments are read before this block's MFMAs (the
-      // scheduling barrier keeps the compiler from sinking the reads to
-      // their use), so the LDS latency overlaps this block's work.
-      gram_bf16x8 n0, n1;
-      load(min(cb + 1, ncb - 1), n0, n1);
-      __builtin_amdgcn_sched_barrier(0);
-      gram_f32x4 acc[RS];
-      block(b0, b1, acc);
-      b0 = n0;
-      b1 = n1;
-#pragma unroll
-      for (int g = 0; g < RS; ++g)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          // Ascending list of the M smallest: the new q-th smallest is the
-          // median of (lst[q-1], lst[q], v); the smallest an unsigned
-          // minimum of the bit patterns (positive bounds or +inf order like
-          // their bits; no NaN: the filter runs on finite norms only).
-          const float v = acc[g][r];
-#pragma unroll
-          for (int q = M - 1; q > 0; --q) mn[g][q] = __builtin_amdgcn_fmed3f(mn[g][q - 1], mn[g][q], v);
-          mn[g][0] = __uint_as_float(min(__float_as_uint(v), __float_as_uint(mn[g][0])));
-        }
-    }
-    // U: the ms-th smallest of the row's 4 M kept values (lanes l & 15 +
-    // 16 k), by ms extractions of the minimum, one instance at a time.
-#pragma unroll
-    for (int g = 0; g < RS; ++g) {
-      float all[4 * M];
-#pragma unroll
-      for (int q = 0; q < M; ++q) {
-        all[q] = mn[g][q];
-        all[M + q] = __shfl_xor(mn[g][q], 16);
-        all[2 * M + q] = __shfl_xor(mn[g][q], 32);
-        all[3 * M + q] = __shfl_xor(mn[g][q], 48);
-      }
-      float u = INFINITY;
-      for (int it = 0; it < ms; ++it) {
-        float m = all[0];
-#pragma unroll
-        for (int q = 1; q < 4 * M; ++q) m = fminf(m, all[q]);
-        bool done = false;
-#pragma unroll
-        for (int q = 0; q < 4 * M; ++q) {
-          const bool hit = !done && all[q] == m;
-          all[q] = hit ? INFINITY : all[q];
-          done = done || hit;
-        }
-        u = m;
-      }
-      U[g] = u;
-    }
-  }
-  // Pass B operands: the lower-bound norm terms replace the upper ones.
-  __syncthreads();  // every wave has finished reading pass A's fragments
-  if (colw) frag[(size_t)(j >> 4) * 128 + 96 + (j & 15)] = norm_slots(fmaf(sqj, -kWinC, sqj) - kGramEh, false, j >= n);
-  if (active && grp == 2) {
-#pragma unroll
-    for (int g = 0; g < RS; ++g)
-      av[g][1] = __builtin_bit_cast(gram_bf16x8, norm_slots(fmaf(sqr[g], -kWinC, sqr[g]) - kGramEh, true, false));
-  }
-  __syncthreads();
-  if (!active) return;
-
-  // Pass B.
-  gram_bf16x8 b0, b1;
-  load(0, b0, b1);
-  for (int c8 = 0; c8 < ncb; c8 += 8) {
-    uint32_t word[RS] = {};
-#pragma unroll
-    for (int cc = 0; cc < 8; ++cc) {
-      const int cb = c8 + cc;
-      if (cb < ncb) {
-        gram_bf16x8 n0, n1;
-        load(min(cb + 1, ncb - 1), n0, n1);
-        __builtin_amdgcn_sched_barrier(0);
-        gram_f32x4 acc[RS];
-        block(b0, b1, acc);
-        b0 = n0;
-        b1 = n1;
-#pragma unroll
-        for (int g = 0; g < RS; ++g)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) word[g] |= (acc[g][r] > U[g] ? 0u : 1u) << (4 * cc + r);
-      }
-    }
-    // Expand the (rare) set bits: column 16 (c8 + b / 4) + 4 grp + b % 4 of
-    // row row0 + 16 g + (lane & 15); rows past the window are never listed.
-#pragma unroll
-    for (int g = 0; g < RS; ++g) {
-      const int i = row0 + 16 * g + (lane & 15);
-      uint32_t m = i < n ? word[g] : 0u;
-      while (m) {
-        const int b = __builtin_ctz(m);
-        m &= m - 1u;
-        const int jc = 16 * (c8 + (b >> 2)) + 4 * grp + (b & 3);
-        if (jc < n) {
-          const uint32_t pos = atomicAdd(&rowcnt[i], 1u);
-          if (pos < (uint32_t)CAP) cand[(base + i) * CAP + pos] = (uint16_t)jc;
-        }
-      }
-    }
-  }
-  // A row's four lanes are in this wave and their counter updates have
-  // completed in program order.
-#pragma unroll
-  for (int g = 0; g < RS; ++g) {
-    const int i = row0 + 16 * g + (lane & 15);
-    if (grp == 0 && i < n) ncand[base + i] = (int32_t)rowcnt[i];
-  }
 }
 
 // The same outputs for DK <= 16 with a block's 256 points staged through
@@ -970,6 +721,304 @@ __global__ __launch_bounds__(1024) void core_exact_lds_kernel(const float* x, Wi
     core[row] = exact_row<KMAX>(xs, nullptr, n, d, ms, i, cand + row * kGramCap, ncand[row], bad);
   }
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+}
+
+// Core distances core[point] of windows of at most kWinMaxN points and 16
+// dimensions, both filter passes and the exact selection in one workgroup
+// per window.  Grid: one workgroup per window of the launch, blockDim.x >=
+// the largest window rounded up to 64, LDS gram_window_lds_bytes(largest
+// window, d).  pairs: per window 16 n 32-bit entries of scratch (the
+// candidate-list buffer of the chunked form, 64 bytes per point).
+//
+// The column fragment is the MFMA's first operand and the row fragment the
+// second, so accumulator register r of lane l holds column 4 (l >> 4) + r of
+// the block and row l & 15 of the set: each lane owns one row per set and
+// sees four of its columns per block (the row's 16 columns of a block are
+// spread over the lanes l & 15 + 16 k, k = 0..3).
+//   Pass A: each lane keeps the M smallest upper bounds of its columns; U =
+//           the ms-th smallest of the 4 M values of the row's four lanes (a
+//           subset of the row, so U >= the ms-th smallest upper bound >= the
+//           exact statistic).  M = 2 for ms <= 8, 4 for ms <= 16: 4 M >= ms.
+//   Pass B: a column is a candidate when its lower bound is <= U.  Each
+//           lane ORs the test results of eight blocks into one mask word
+//           (bit 4 c + r: block c of the eight, register r) and only set
+//           bits are expanded: the row's LDS counter counts the candidate,
+//           and while the count is below kGramCap the pair (row << 16 |
+//           column) is appended to the window's pair array (one LDS counter
+//           per window): consecutive 4-byte stores, not scattered list
+//           slots.  A pair past the array's capacity marks its row for the
+//           exact path (bit 31 of its counter).
+//   Exact:  the fragments are dead; the window's points are staged in LDS,
+//           the pairs are sorted into per-row lists in LDS, and each thread
+//           selects its row's core distance from its list, or from the
+//           whole window when the row had more than kGramCap candidates
+//           (exact_row, as core_exact_lds_kernel).
+// M: values kept per lane in pass A (2, 4); KMAX: the exact selection's
+// list length (8 for ms <= 8, 16 for ms <= 16).
+template <int M, int KMAX>
+__global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windows win, int d, int ms,
+                                                           uint32_t* pairs, float* core, int32_t* status) {
+  constexpr int DK = kWinDK, CAP = kGramCap, RS = 4;
+  constexpr uint32_t kFull = 0x80000000u;  // row counter bit: exact path
+  extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ uint32_t sh_off, sh_pairs;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int grp = lane >> 4;
+  int64_t base;
+  int n;
+  int wid;
+  win.get(blockIdx.x, wid, base, n);
+  const float* xb = x + (size_t)base * d;
+  const int ncb = (n + 15) >> 4;
+  uint32_t* rowcnt = reinterpret_cast<uint32_t*>(lds);         // [n] candidates per row
+  uint32_t* fillcnt = rowcnt + n;                               // [n] list fill (exact stage)
+  unsigned char* region = lds + gram_window_head(n);
+  gram_s16x8* frag = reinterpret_cast<gram_s16x8*>(region);    // [cb][q][64 lanes]
+  uint32_t* wpairs = pairs + (size_t)base * 16;
+  const uint32_t cap = 16u * (uint32_t)n;
+  if (tid == 0) sh_off = sh_pairs = 0;
+  if (tid < n) rowcnt[tid] = fillcnt[tid] = 0;
+  __syncthreads();
+
+  // Column operands: thread j writes column j's eight 16-byte entries
+  // [hi(-2x) | lo(-2x) | hi(-2x) | norm slots | zeros].  One column per
+  // thread: blockDim.x >= 64 ceil(n / 64) >= 16 ncb.
+  int bad = 0;
+  float sqj = 0.f;
+  const int j = tid;
+  const bool colw = j < 16 * ncb;
+  if (colw) {
+    gram_s16x8 bh[2] = {}, bl[2] = {};
+    if (j < n) {
+      float v[DK];
+      window_load(xb + (size_t)j * d, d, v);
+      sqj = window_norm(v, bad);
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float w = -2.f * v[8 * h + e];
+          const uint32_t g = bf16_bits(w);
+          bh[h][e] = (short)g;
+          bl[h][e] = (short)bf16_bits(w - bf16_value(g));
+        }
+      // NaN compares false: a non-finite norm turns the filter off too.
+      if (!(sqj <= 1.2676506002282294e30f)) atomicOr(&sh_off, 1u);
+    }
+    const int cl = j & 15;
+    gram_s16x8* f0 = frag + (size_t)(j >> 4) * 128 + cl;
+    f0[0] = bh[0];
+    f0[16] = bh[1];
+    f0[32] = bl[0];
+    f0[48] = bl[1];
+    f0[64] = bh[0];
+    f0[80] = bh[1];
+    f0[96] = norm_slots(fmaf(sqj, kWinC, sqj) + kGramEh, false, j >= n);
+    f0[112] = gram_s16x8{};
+  }
+  if (__any(bad) && lane == 0) atomicOr(status, 2);
+  __syncthreads();
+  const int row0 = wave * 16 * RS;
+  const bool active = row0 < n;
+  if (sh_off) {
+    // Filter off for this window: every row goes to the exact path.
+    if (tid < n) rowcnt[tid] = kFull;
+  } else {
+
+  // Row operands of set g: row row0 + 16 g + (lane & 15) as
+  // [hi(x) | hi(x) | lo(x) | norm slots | zeros] (lane holds K entries
+  // 8 grp .. 8 grp + 7 of each 32-wide MFMA step).
+  gram_bf16x8 av[RS][2];
+  float sqr[RS];
+  if (active) {
+    // All four rows' loads are issued before any arithmetic.
+    float vr[RS][DK];
+#pragma unroll
+    for (int g = 0; g < RS; ++g) window_load(xb + (size_t)min(row0 + 16 * g + (lane & 15), n - 1) * d, d, vr[g]);
+#pragma unroll
+    for (int g = 0; g < RS; ++g) {
+      const float (&v)[DK] = vr[g];
+      int ignore = 0;
+      sqr[g] = window_norm(v, ignore);
+      const int h = grp & 1;
+      gram_s16x8 hi, lo;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float c = v[8 * h + e];
+        const uint32_t b = bf16_bits(c);
+        hi[e] = (short)b;
+        lo[e] = (short)bf16_bits(c - bf16_value(b));
+      }
+      av[g][0] = __builtin_bit_cast(gram_bf16x8, hi);
+      const gram_s16x8 q1 = grp < 2 ? lo : grp == 2 ? norm_slots(fmaf(sqr[g], kWinC, sqr[g]) + kGramEh, true, false)
+                                                    : gram_s16x8{};
+      av[g][1] = __builtin_bit_cast(gram_bf16x8, q1);
+    }
+  }
+
+  // Bounds of one column block for the four sets: all eight MFMAs issue
+  // before any result is read.
+  auto block = [&](const gram_bf16x8& b0, const gram_bf16x8& b1, gram_f32x4 (&acc)[RS]) {
+#pragma unroll
+    for (int g = 0; g < RS; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, av[g][0], gram_f32x4{}, 0, 0, 0);
+#pragma unroll
+    for (int g = 0; g < RS; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, av[g][1], acc[g], 0, 0, 0);
+  };
+  auto load = [&](int cb, gram_bf16x8& b0, gram_bf16x8& b1) {
+    b0 = __builtin_bit_cast(gram_bf16x8, frag[cb * 128 + lane]);
+    b1 = __builtin_bit_cast(gram_bf16x8, frag[cb * 128 + 64 + lane]);
+  };
+
+  float U[RS];
+  if (active) {
+    // Pass A.
+    float mn[RS][M];
+#pragma unroll
+    for (int g = 0; g < RS; ++g)
+#pragma unroll
+      for (int q = 0; q < M; ++q) mn[g][q] = INFINITY;
+    gram_bf16x8 b0, b1;
+    load(0, b0, b1);
+    for (int cb = 0; cb < ncb; ++cb) {
+      // The next block's fragments are read before this block's MFMAs (the
+      // scheduling barrier keeps the compiler from sinking the reads to
+      // their use), so the LDS latency overlaps this block's work.
+      gram_bf16x8 n0, n1;
+      load(min(cb + 1, ncb - 1), n0, n1);
+      __builtin_amdgcn_sched_barrier(0);
+      gram_f32x4 acc[RS];
+      block(b0, b1, acc);
+      b0 = n0;
+      b1 = n1;
+#pragma unroll
+      for (int g = 0; g < RS; ++g)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          // Ascending list of the M smallest: the new q-th smallest is the
+          // median of (lst[q-1], lst[q], v); the smallest an unsigned
+          // minimum of the bit patterns (positive bounds or +inf order like
+          // their bits; no NaN: the filter runs on finite norms only).
+          const float v = acc[g][r];
+#pragma unroll
+          for (int q = M - 1; q > 0; --q) mn[g][q] = __builtin_amdgcn_fmed3f(mn[g][q - 1], mn[g][q], v);
+          mn[g][0] = __uint_as_float(min(__float_as_uint(v), __float_as_uint(mn[g][0])));
+        }
+    }
+    // U: the ms-th smallest of the row's 4 M kept values (lanes l & 15 +
+    // 16 k), by ms extractions of the minimum, one instance at a time.
+#pragma unroll
+    for (int g = 0; g < RS; ++g) {
+      float all[4 * M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        all[q] = mn[g][q];
+        all[M + q] = __shfl_xor(mn[g][q], 16);
+        all[2 * M + q] = __shfl_xor(mn[g][q], 32);
+        all[3 * M + q] = __shfl_xor(mn[g][q], 48);
+      }
+      float u = INFINITY;
+      for (int it = 0; it < ms; ++it) {
+        float m = all[0];
+#pragma unroll
+        for (int q = 1; q < 4 * M; ++q) m = fminf(m, all[q]);
+        bool done = false;
+#pragma unroll
+        for (int q = 0; q < 4 * M; ++q) {
+          const bool hit = !done && all[q] == m;
+          all[q] = hit ? INFINITY : all[q];
+          done = done || hit;
+        }
+        u = m;
+      }
+      U[g] = u;
+    }
+  }
+  // Pass B operands: the lower-bound norm terms replace the upper ones.
+  __syncthreads();  // every wave has finished reading pass A's fragments
+  if (colw) frag[(size_t)(j >> 4) * 128 + 96 + (j & 15)] = norm_slots(fmaf(sqj, -kWinC, sqj) - kGramEh, false, j >= n);
+  if (active && grp == 2) {
+#pragma unroll
+    for (int g = 0; g < RS; ++g)
+      av[g][1] = __builtin_bit_cast(gram_bf16x8, norm_slots(fmaf(sqr[g], -kWinC, sqr[g]) - kGramEh, true, false));
+  }
+  __syncthreads();
+  if (active) {
+  // Pass B.
+  gram_bf16x8 b0, b1;
+  load(0, b0, b1);
+  for (int c8 = 0; c8 < ncb; c8 += 8) {
+    uint32_t word[RS] = {};
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc) {
+      const int cb = c8 + cc;
+      if (cb < ncb) {
+        gram_bf16x8 n0, n1;
+        load(min(cb + 1, ncb - 1), n0, n1);
+        __builtin_amdgcn_sched_barrier(0);
+        gram_f32x4 acc[RS];
+        block(b0, b1, acc);
+        b0 = n0;
+        b1 = n1;
+#pragma unroll
+        for (int g = 0; g < RS; ++g)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) word[g] |= (acc[g][r] > U[g] ? 0u : 1u) << (4 * cc + r);
+      }
+    }
+    // Expand the (rare) set bits: column 16 (c8 + b / 4) + 4 grp + b % 4 of
+    // row row0 + 16 g + (lane & 15); rows past the window are never listed.
+#pragma unroll
+    for (int g = 0; g < RS; ++g) {
+      const int i = row0 + 16 * g + (lane & 15);
+      uint32_t m = i < n ? word[g] : 0u;
+      while (m) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1u;
+        const int jc = 16 * (c8 + (b >> 2)) + 4 * grp + (b & 3);
+        if (jc < n) {
+          const uint32_t pos = atomicAdd(&rowcnt[i], 1u);
+          if (pos < (uint32_t)CAP) {
+            const uint32_t k = atomicAdd(&sh_pairs, 1u);
+            if (k < cap)
+              wpairs[k] = ((uint32_t)i << 16) | (uint32_t)jc;
+            else
+              atomicOr(&rowcnt[i], kFull);
+          }
+        }
+      }
+    }
+  }
+  }  // active
+  }  // filter on
+  __syncthreads();  // every pair is appended; the fragments are dead
+
+  // Exact selection.  The pairs were written by this workgroup before the
+  // barrier, so its reads see them.
+  float* xs = reinterpret_cast<float*>(region);  // [n][d]
+  uint16_t* lists = reinterpret_cast<uint16_t*>(region + ((4u * (uint32_t)n * (uint32_t)d + 15u) & ~15u));
+  const int total = n * d;
+  if (((((uintptr_t)xb) & 15) == 0) && (total & 3) == 0) {
+    for (int q = tid; q < total / 4; q += blockDim.x)
+      reinterpret_cast<float4*>(xs)[q] = reinterpret_cast<const float4*>(xb)[q];
+  } else {
+    for (int q = tid; q < total; q += blockDim.x) xs[q] = xb[q];
+  }
+  const uint32_t np = min(sh_pairs, cap);
+  for (uint32_t q = tid; q < np; q += blockDim.x) {
+    const uint32_t pr = wpairs[q];
+    const uint32_t i = pr >> 16;
+    const uint32_t slot = atomicAdd(&fillcnt[i], 1u);  // < kGramCap: appended only below it
+    lists[i * CAP + slot] = (uint16_t)(pr & 0xFFFFu);
+  }
+  __syncthreads();
+  int badx = 0;
+  for (int i = tid; i < n; i += blockDim.x) {
+    const uint32_t rc = rowcnt[i];
+    const int m = (rc & kFull) ? CAP + 1 : (int)rc;
+    core[base + i] = exact_row<KMAX>(xs, nullptr, n, d, ms, i, lists + i * CAP, m, badx);
+  }
+  if (__any(badx) && lane == 0) atomicOr(status, 1);
 }
 
 }  // namespace hdb

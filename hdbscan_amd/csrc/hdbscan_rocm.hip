@@ -253,7 +253,7 @@ void set_gram(Plan& P, int64_t n, int64_t d, int32_t ms, uint32_t lds_limit) {
   // Windows of up to 1024 points and 16 dimensions: one workgroup per window
   // with the column operands in LDS once, no prep kernel.
   P.gram_window = dk <= kWinDK && n <= kWinMaxN && !P.matrix && !(P.flags & HDB_FLAG_GRAM_CHUNKED) &&
-                  gram_window_lds_bytes((int)n) <= lds_limit;
+                  gram_window_lds_bytes((int)n, (int)d) <= lds_limit;
   P.gram_dk = dk;
   P.gram_rs = rs;
   P.gram_threads = waves * 64;
@@ -593,13 +593,15 @@ int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist,
                 int ms, float* core, unsigned char* ws, int32_t* status, bool prep) {
   const GramBuffers g = gram_buffers(P, ws);
   if (P.gram_window && !dist) {
+    // Both passes and the exact selection in one workgroup per window; the
+    // candidate-list buffer holds each window's (row, column) pairs.
     const int threads = (int)std::min<int64_t>(1024, (L.nmax + 63) / 64 * 64);
-    const uint32_t lds = gram_window_lds_bytes((int)L.nmax);
-    auto fn = ms <= 8 ? core_window_kernel<2> : core_window_kernel<4>;
+    const uint32_t lds = gram_window_lds_bytes((int)L.nmax, (int)d);
+    auto fn = ms <= 8 ? core_window_kernel<2, 8> : core_window_kernel<4, 16>;
     if (int r = allow_lds((const void*)fn, lds)) return r;
-    fn<<<(unsigned)L.count, threads, lds, s>>>(x, L.win, (int)d, ms, g.cand, g.ncand, status);
-    if (int r = check_launch("core_window_kernel")) return r;
-    stage_mark(s, "core_window_kernel");
+    fn<<<(unsigned)L.count, threads, lds, s>>>(x, L.win, (int)d, ms, reinterpret_cast<uint32_t*>(g.cand), core,
+                                              status);
+    return check_launch("core_window_kernel");
   } else {
     if (prep) {
       if (int r = launch_gram_prep(s, P, x, L.points, d, ws, status)) return r;
@@ -898,7 +900,8 @@ int run_cluster(hipStream_t s, const Plan& P, const Real* x, const Real* given_d
     if (!unit) {
       if (int r = launch_core<Real>(s, P, x, L, d, ms, core, matrix, ws, status, matrix_ready, gram_prep)) return r;
       core_in = core;
-      stage_mark(s, P.core_gram ? "core_exact_kernel" : matrix_ready ? "core_select_kernel" : "core_topk_kernel");
+      stage_mark(s, P.core_gram ? (P.gram_window && !matrix_ready ? "core_window_kernel" : "core_exact_kernel")
+                                : matrix_ready ? "core_select_kernel" : "core_topk_kernel");
     }
   }
   if (core_done && hipEventRecord(core_done, s) != hipSuccess) return fail("event record failed");
@@ -1513,7 +1516,8 @@ int hdb_core_distances(void* stream, const void* x, int64_t batch, int64_t n, in
   P.kmax = min_samples <= 8 ? 8 : min_samples <= 16 ? 16 : min_samples <= 32 ? 32 : 64;
   // No matrix is materialised before the core stage here, so the Gram filter
   // takes its per-window form where the clustering path would.
-  P.gram_window = P.core_gram && P.gram_dk <= kWinDK && n <= kWinMaxN && !(flags & HDB_FLAG_GRAM_CHUNKED);
+  P.gram_window = P.core_gram && P.gram_dk <= kWinDK && n <= kWinMaxN && !(flags & HDB_FLAG_GRAM_CHUNKED) &&
+                  gram_window_lds_bytes((int)n, (int)d) <= device_lds_limit();
   const size_t bn = (size_t)batch * n;
   const size_t need = P.core_gram ? P.total : P.core_topk ? 0 : P.off_matrix + bn * n * P.rs;
   if (workspace_bytes < need || (need && !workspace)) return fail("workspace too small");

@@ -111,6 +111,7 @@ def test_json_line_schema():
 def test_roofline_kinds():
     assert bench.roofline_for("prim_kernel", 3.0, 128, 2048, 128, "float32", matrix=True)["bound"] == "hbm"
     assert bench.roofline_for("core_gram_kernel", 0.1, 256, 1024, 16, "float32")["bound"] == "mfma"
+    assert bench.roofline_for("core_window_kernel", 0.1, 256, 1024, 16, "float32")["bound"] == "mfma"
     assert bench.roofline_for("tree_kernel", 0.2, 256, 1024, 16, "float32")["bound"] == "hbm"
     r = bench.roofline_for("distance_kernel", 2.5, 128, 2048, 128, "float32", valu_tops=42.0)
     assert r["bound"] == "valu" and r["unit"] == "TFLOP/s"
