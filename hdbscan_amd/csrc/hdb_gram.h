@@ -476,10 +476,13 @@ __device__ __forceinline__ gram_s16x8 norm_slots(float v, bool row_side, bool in
   return s;
 }
 
-// Loads a point's DK padded coordinates (zeros past d): four 16-byte loads
-// when d == DK and the row is 16-byte aligned.
+// Loads a point's DK padded coordinates (zeros past d).  VEC (d == DK and
+// every row 16-byte aligned): four 16-byte loads.  One path per
+// instantiation: with both in one function the compiler keeps the array in
+// scratch memory.
+template <bool VEC>
 __device__ __forceinline__ void window_load(const float* xp, int d, float (&v)[kWinDK]) {
-  if (d == kWinDK && (reinterpret_cast<uintptr_t>(xp) & 15) == 0) {
+  if constexpr (VEC) {
 #pragma unroll
     for (int q = 0; q < kWinDK / 4; ++q) {
       const float4 t = reinterpret_cast<const float4*>(xp)[q];
@@ -754,8 +757,9 @@ __global__ __launch_bounds__(1024) void core_exact_lds_kernel(const float* x, Wi
 //           whole window when the row had more than kGramCap candidates
 //           (exact_row, as core_exact_lds_kernel).
 // M: values kept per lane in pass A (2, 4); KMAX: the exact selection's
-// list length (8 for ms <= 8, 16 for ms <= 16).
-template <int M, int KMAX>
+// list length (8 for ms <= 8, 16 for ms <= 16); VEC: 16-dimensional rows
+// read as 16-byte vectors (the input 16-byte aligned).
+template <int M, int KMAX, bool VEC>
 __global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windows win, int d, int ms,
                                                            uint32_t* pairs, float* core, int32_t* status) {
   constexpr int DK = kWinDK, CAP = kGramCap, RS = 4;
@@ -792,7 +796,7 @@ __global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windo
     gram_s16x8 bh[2] = {}, bl[2] = {};
     if (j < n) {
       float v[DK];
-      window_load(xb + (size_t)j * d, d, v);
+      window_load<VEC>(xb + (size_t)j * d, d, v);
       sqj = window_norm(v, bad);
 #pragma unroll
       for (int h = 0; h < 2; ++h)
@@ -835,24 +839,30 @@ __global__ __launch_bounds__(1024) void core_window_kernel(const float* x, Windo
     // All four rows' loads are issued before any arithmetic.
     float vr[RS][DK];
 #pragma unroll
-    for (int g = 0; g < RS; ++g) window_load(xb + (size_t)min(row0 + 16 * g + (lane & 15), n - 1) * d, d, vr[g]);
+    for (int g = 0; g < RS; ++g) window_load<VEC>(xb + (size_t)min(row0 + 16 * g + (lane & 15), n - 1) * d, d, vr[g]);
 #pragma unroll
     for (int g = 0; g < RS; ++g) {
       const float (&v)[DK] = vr[g];
       int ignore = 0;
       sqr[g] = window_norm(v, ignore);
-      const int h = grp & 1;
-      gram_s16x8 hi, lo;
+      // Both halves are split and the lane's half selected as a vector: an
+      // index that depends on the lane (v[8 (grp & 1) + e]) would put the
+      // coordinates in scratch memory.
+      gram_s16x8 hi0, hi1, lo0, lo1;  // named values, not an array the select could index
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const float c = v[8 * h + e];
-        const uint32_t b = bf16_bits(c);
-        hi[e] = (short)b;
-        lo[e] = (short)bf16_bits(c - bf16_value(b));
+        const uint32_t b0 = bf16_bits(v[e]), b1 = bf16_bits(v[8 + e]);
+        hi0[e] = (short)b0;
+        lo0[e] = (short)bf16_bits(v[e] - bf16_value(b0));
+        hi1[e] = (short)b1;
+        lo1[e] = (short)bf16_bits(v[8 + e] - bf16_value(b1));
       }
-      av[g][0] = __builtin_bit_cast(gram_bf16x8, hi);
-      const gram_s16x8 q1 = grp < 2 ? lo : grp == 2 ? norm_slots(fmaf(sqr[g], kWinC, sqr[g]) + kGramEh, true, false)
-                                                    : gram_s16x8{};
+      const bool h1 = (grp & 1) != 0;
+      av[g][0] = __builtin_bit_cast(gram_bf16x8, h1 ? hi1 : hi0);
+      const gram_s16x8 lo_h = h1 ? lo1 : lo0;
+      const gram_s16x8 q1 = grp < 2    ? lo_h
+                            : grp == 2 ? norm_slots(fmaf(sqr[g], kWinC, sqr[g]) + kGramEh, true, false)
+                                       : gram_s16x8{};
       av[g][1] = __builtin_bit_cast(gram_bf16x8, q1);
     }
   }

@@ -597,7 +597,9 @@ int launch_gram(hipStream_t s, const Plan& P, const float* x, const float* dist,
     // candidate-list buffer holds each window's (row, column) pairs.
     const int threads = (int)std::min<int64_t>(1024, (L.nmax + 63) / 64 * 64);
     const uint32_t lds = gram_window_lds_bytes((int)L.nmax, (int)d);
-    auto fn = ms <= 8 ? core_window_kernel<2, 8> : core_window_kernel<4, 16>;
+    const bool vec = d == kWinDK && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    auto fn = ms <= 8 ? (vec ? core_window_kernel<2, 8, true> : core_window_kernel<2, 8, false>)
+                      : (vec ? core_window_kernel<4, 16, true> : core_window_kernel<4, 16, false>);
     if (int r = allow_lds((const void*)fn, lds)) return r;
     fn<<<(unsigned)L.count, threads, lds, s>>>(x, L.win, (int)d, ms, reinterpret_cast<uint32_t*>(g.cand), core,
                                               status);
