@@ -86,13 +86,20 @@ def bounded_distances(x, min_samples):
     return host(out), int(host(status)[0])
 
 
-def core(x, min_samples, precision, flags=0):
+def core(x, min_samples, precision, flags=0, misalign=False):
+    """misalign: the points start one element past an allocation's start
+    (a pointer that is not 16-byte aligned)."""
     t = torch()
     x = np.asarray(x, real(precision))
     if x.ndim == 2:
         x = x[None]
     b, n, d = x.shape
     xd = dev(x)
+    xptr = ptr(xd)
+    if misalign:
+        flat = t.empty(x.size + 1, dtype=xd.dtype, device="cuda")
+        flat[1:] = xd.reshape(-1)
+        xptr = P(flat.data_ptr() + flat.element_size())
     out = t.empty((b, n), dtype=xd.dtype, device="cuda")
     status = t.zeros(1, dtype=t.int32, device="cuda")
     ws_bytes = _native.lib().hdb_workspace_bytes(b, n, d, code(precision), int(min_samples),
@@ -100,7 +107,7 @@ def core(x, min_samples, precision, flags=0):
                                                  | (0 if flags & 64 else 128))
     assert ws_bytes > 0, _native.lib().hdb_last_error()
     ws = t.empty(ws_bytes, dtype=t.uint8, device="cuda")
-    _native.check(_native.lib().hdb_core_distances(stream(), ptr(xd), b, n, d, code(precision), int(min_samples),
+    _native.check(_native.lib().hdb_core_distances(stream(), xptr, b, n, d, code(precision), int(min_samples),
                                                    ptr(out), ptr(ws), ws_bytes, ptr(status), int(flags)))
     return host(out), int(host(status)[0])
 

@@ -103,6 +103,29 @@ def test_core_gram_filter_bitwise(ms, rsets, chunked):
         np.testing.assert_array_equal(got[0].view(np.uint32), exp.view(np.uint32), err_msg=name)
 
 
+@pytest.mark.parametrize("ms", [5, 16])
+def test_core_window_rare_paths_bitwise(ms):
+    """The per-window Gram kernel's rarely taken paths: scalar row loads (a
+    16-dimensional input that is not 16-byte aligned), rows whose candidates
+    overflow the window's pair array (clusters of 20 exact duplicates: more
+    than 16 candidates per row on average, so later rows take the
+    whole-row path), and a window of one repeated point (every row past 32
+    candidates)."""
+    rng = np.random.default_rng(77)
+    cases = {
+        "misaligned": (rng.normal(size=(1000, 16)), True),
+        "dup20": (np.repeat(rng.normal(size=(50, 16)) * 4, 20, axis=0), False),
+        "same": (np.ones((300, 16)), False),
+        "dup20-d7": (np.repeat(rng.normal(size=(40, 7)), 20, axis=0), False),
+    }
+    for name, (x, mis) in cases.items():
+        x = x.astype(np.float32)
+        got, status = G.core(x, ms, "float32", 0, misalign=mis)
+        assert status == 0, name
+        exp = oracle.core(oracle.distances(x, "float32"), ms, "float32")
+        np.testing.assert_array_equal(got[0].view(np.uint32), exp.view(np.uint32), err_msg=name)
+
+
 def path_graph(weights, dtype):
     size = len(weights) + 1
     dist = np.full((size, size), size * 10, dtype=dtype)
