@@ -727,7 +727,7 @@ __device__ __forceinline__ void prim_window(const PrimArgs& a, int item) {
 
 // One window per workgroup, or windows claimed from the launch's queue.
 template <typename Real, int VPT, int DK, int MODE, bool XLDS>
-__global__ __launch_bounds__(1024) void prim_kernel(PrimArgs a) {
+__global__ __launch_bounds__(VPT == 8 ? 512 : 1024) void prim_kernel(PrimArgs a) {
   for_each_item(a.win, [&](int item) { prim_window<Real, VPT, DK, MODE, XLDS>(a, item); });
 }
 

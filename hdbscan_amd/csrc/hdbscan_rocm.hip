@@ -309,9 +309,14 @@ int make_plan(int64_t batch, int64_t n, int64_t d, int dtype, int32_t ms, int fl
   }
   while ((int64_t)vpt * 1024 < n) vpt *= 2;
   if (vpt > 4) return fail("window too large for the vertex layout");
+  // Points path above 2048 points, at most 16 float32 coordinates: eight
+  // slots on 512 threads.  Four slots' coordinates and state on 1024 threads
+  // exceed the 128 registers such a workgroup allows, and the compiler
+  // spills them to scratch inside the step loop; 512 threads allow 256.
+  if (vpt == 4 && want_vpt != 4 && rs == 4 && dk > 0 && dk <= 16 && !(flags & HDB_FLAG_MATRIX)) vpt = 8;
   P.vpt = vpt;
   P.threads = (int)(((n + vpt - 1) / vpt + 63) / 64 * 64);
-  P.matrix = (flags & HDB_FLAG_MATRIX) || dk == 0 || (int64_t)vpt * dk * rs > 64 * 4;
+  P.matrix = (flags & HDB_FLAG_MATRIX) || dk == 0 || (int64_t)vpt * dk * rs > (vpt == 8 ? 64 * 8 : 64 * 4);
   // The speculative matrix-row Prim repeats its reductions in every wave:
   // 8 vertices per thread halves the waves of a window (no coordinates are
   // held in registers on this path).
@@ -684,7 +689,7 @@ int launch_prim_spec(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs
 
 template <typename Real, int VPT, int DK>
 int launch_prim_x(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) {
-  if constexpr (VPT * DK * sizeof(Real) <= 64 * 4) {
+  if constexpr (VPT * DK * sizeof(Real) <= (VPT == 8 ? 64 * 8 : 64 * 4)) {
     if (a.prefilter) {
       if (P.xlds) return launch_prim_one<Real, VPT, DK, 2, true>(s, P, batch, a);
       return launch_prim_one<Real, VPT, DK, 2, false>(s, P, batch, a);
@@ -716,7 +721,13 @@ int launch_prim(hipStream_t s, const Plan& P, int64_t batch, const PrimArgs& a) 
     case 1: return launch_prim_v<Real, 1>(s, P, batch, a);
     case 2: return launch_prim_v<Real, 2>(s, P, batch, a);
     case 4: return launch_prim_v<Real, 4>(s, P, batch, a);
-    case 8: return launch_prim_spec<Real, 8>(s, P, batch, a);
+    case 8:
+      if (P.matrix) return launch_prim_spec<Real, 8>(s, P, batch, a);
+      if constexpr (sizeof(Real) == 4) {
+        if (P.dk == 8) return launch_prim_x<Real, 8, 8>(s, P, batch, a);
+        if (P.dk == 16) return launch_prim_x<Real, 8, 16>(s, P, batch, a);
+      }
+      return fail("internal: unsupported points layout");
   }
   return fail("internal: unsupported vertices per thread");
 }
